@@ -1,0 +1,134 @@
+#pragma once
+
+// Device helpers shared by the HIP kernels (gfx950, wave64).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hnswamd {
+namespace gpu {
+
+constexpr int kWave = 64;
+constexpr uint64_t kKeyInf = ~0ULL;
+constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
+constexpr uint32_t kOrdInf = 0xFFFFFFFFu;  // larger than the key of any non-NaN distance
+
+// fp32 distance -> u32 whose unsigned order is the float order (no NaN).
+__device__ __forceinline__ uint32_t ord_of(float d) {
+  const uint32_t u = __float_as_uint(d);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float float_of(uint32_t o) {
+  const uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+  return __uint_as_float(u);
+}
+// (distance, id) packed so that u64 order is the (distance, id) order.
+__device__ __forceinline__ uint64_t key_of(uint32_t ord, uint32_t id) {
+  return (static_cast<uint64_t>(ord) << 32) | id;
+}
+__device__ __forceinline__ uint32_t key_ord(uint64_t k) { return static_cast<uint32_t>(k >> 32); }
+__device__ __forceinline__ uint32_t key_id(uint64_t k) { return static_cast<uint32_t>(k); }
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, int lane) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
+}
+__device__ __forceinline__ uint64_t rdlane64(uint64_t v, int lane) {
+  const uint32_t lo = rdlane(static_cast<uint32_t>(v), lane);
+  const uint32_t hi = rdlane(static_cast<uint32_t>(v >> 32), lane);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = uniform(static_cast<uint32_t>(v));
+  const uint32_t hi = uniform(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Value of lane (lane - 1) / (lane + 1) through the LDS crossbar (no memory).
+__device__ __forceinline__ uint32_t from_lane(uint32_t v, int src_lane) {
+  return static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src_lane << 2, static_cast<int>(v)));
+}
+__device__ __forceinline__ uint64_t from_lane64(uint64_t v, int src_lane) {
+  const uint32_t lo = from_lane(static_cast<uint32_t>(v), src_lane);
+  const uint32_t hi = from_lane(static_cast<uint32_t>(v >> 32), src_lane);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Quad permutes through DPP: partner lane (lane ^ 1) / (lane ^ 2) within each
+// group of four lanes.
+__device__ __forceinline__ float quad_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float quad_xor2(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+
+/// A sorted ascending array of u64 keys spread over the wave: element k lives in
+/// row k / 64 of lane k % 64. Empty slots hold kKeyInf. Every member is called
+/// by the whole wave with uniform arguments.
+template <int E>
+struct WaveSorted {
+  uint64_t v[E];
+
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = kKeyInf;
+  }
+  __device__ __forceinline__ uint32_t rank(uint64_t x) const {
+    uint32_t c = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) c += static_cast<uint32_t>(__popcll(ballot(v[e] < x)));
+    return c;
+  }
+  __device__ __forceinline__ uint64_t at(uint32_t k) const {
+    uint64_t r = kKeyInf;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (static_cast<uint32_t>(e) == (k >> 6)) r = rdlane64(v[e], static_cast<int>(k & 63));
+    }
+    return r;
+  }
+  /// Insert x keeping the first `cap` (<= 64 * E) elements. Returns the element
+  /// that no longer fits (x itself, or the old element at cap - 1), or kKeyInf.
+  __device__ __forceinline__ uint64_t insert(uint64_t x, uint32_t cap) {
+    const uint32_t pos = rank(x);
+    if (pos >= cap) return x;
+    const uint64_t out = at(cap - 1);
+    const int lane = lane_id();
+#pragma unroll
+    for (int e = E - 1; e >= 0; --e) {
+      const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
+      uint64_t up = from_lane64(v[e], lane == 0 ? 0 : lane - 1);
+      if (e > 0) {
+        const uint64_t carry = rdlane64(v[e - 1], 63);
+        if (lane == 0) up = carry;
+      }
+      uint64_t nv = v[e];
+      if (k == pos) nv = x;
+      if (k > pos) nv = up;
+      if (k >= cap) nv = kKeyInf;
+      v[e] = nv;
+    }
+    return out;
+  }
+  /// Remove element 0.
+  __device__ __forceinline__ void pop_front() {
+    const int lane = lane_id();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      uint64_t dn = from_lane64(v[e], lane == 63 ? 63 : lane + 1);
+      uint64_t carry = kKeyInf;
+      if (e + 1 < E) carry = rdlane64(v[e + 1 < E ? e + 1 : e], 0);
+      if (lane == 63) dn = carry;
+      v[e] = dn;
+    }
+  }
+};
+
+}  // namespace gpu
+}  // namespace hnswamd

@@ -1,0 +1,474 @@
+// The device layer of the C ABI (include/hnswgpu.h, hnswgpu_*): uploads a
+// loaded index into HBM once and serves search_batch from it.
+//
+// HBM layout of one index (all arrays separate allocations, 256-byte aligned):
+//   vecs       count x padded fp32       rows of padded*4 bytes (a multiple of 64)
+//   l0         count x (1 + 2M) u32      [count][ids...] per node
+//   upper      upper_used u32            per node of level L: L blocks of (1 + M)
+//   upper_off  count u32                 prefix sum of level * (1 + M)
+//   labels     count u64
+//   deleted    ceil(count / 32) u32      bit set = tombstone
+// plus per-handle scratch grown on demand (prepared queries, eligibility
+// bitmap, overflow queue, counters, general-kernel heaps).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/hnswgpu.h"
+#include "../error.hpp"
+#include "launch.hpp"
+
+using hnswamd::gpu::DevIndexView;
+using hnswamd::gpu::OutRow;
+using hnswamd::gpu::SearchRequest;
+using hnswamd::gpu::SearchScratch;
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw hnswamd::DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+/// A device allocation that grows (discarding contents) when asked for more.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* need(size_t n) {
+    if (n <= bytes && p != nullptr) return p;
+    if (p != nullptr) {
+      (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    const size_t alloc = std::max<size_t>(n, 256);
+    check(hipMalloc(&p, alloc), "hipMalloc");
+    bytes = alloc;
+    return p;
+  }
+  void release() {
+    if (p != nullptr) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+uint32_t next_log2(uint64_t v) {
+  uint32_t l = 0;
+  while ((1ULL << l) < v) ++l;
+  return l;
+}
+
+}  // namespace
+
+struct hnswgpu_index {
+  std::mutex mu;
+  DevIndexView view{};
+  uint64_t dim = 0;
+  uint64_t ef_search = 10;
+  uint64_t num_deleted = 0;
+  uint64_t upper_used = 0;
+  std::vector<uint8_t> deleted_host;  // mirror of the device bits, for counting
+
+  DevBuf vecs, l0, upper, upper_off, labels, deleted;
+  // scratch
+  DevBuf counters, overflow, qprep, elig, spill_bits, spill_heaps, host_qs, host_out, host_counts, host_filter;
+  uint32_t spill_waves = 64;
+  uint64_t spill_alloc_count = 0;
+
+  // options
+  uint32_t hash_log2 = 0;
+  int force_spill = 0;
+  int collect_stats = 0;
+  int kernel_timing = 0;
+  uint32_t last_overflow = 0;
+  uint64_t last_nq = 0;
+  hipStream_t stream = nullptr;
+  DevBuf stats;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  bool timing_pending = false;
+
+  ~hnswgpu_index() {
+    for (DevBuf* b : {&vecs, &l0, &upper, &upper_off, &labels, &deleted, &counters, &overflow, &qprep, &elig,
+                      &spill_bits, &spill_heaps, &host_qs, &host_out, &host_counts, &host_filter}) {
+      b->release();
+    }
+    stats.release();
+    if (ev_begin != nullptr) (void)hipEventDestroy(ev_begin);
+    if (ev_end != nullptr) (void)hipEventDestroy(ev_end);
+    if (stream != nullptr) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void upload_view(hnswgpu_index& h, const hnswgpu_index_view& v) {
+  if (v.dim == 0) throw std::invalid_argument("upload: dim must be > 0");
+  if (v.padded_dim != (v.dim + 15) / 16 * 16) throw std::invalid_argument("upload: padded_dim must be dim rounded up to 16");
+  if (v.padded_dim > 4096) throw std::invalid_argument("upload: GPU search supports padded_dim <= 4096");
+  if (v.M < 2 || 2 * v.M > hnswamd::gpu::kMaxDegree0) {
+    throw std::invalid_argument("upload: GPU search supports 2 <= M <= 32");
+  }
+  if (v.metric < 0 || v.metric > 2) throw std::invalid_argument("upload: unknown metric");
+  if (v.count >= 0xFFFFFFFFull) throw std::invalid_argument("upload: count exceeds the 32-bit id space");
+  if (v.upper_used >= 0xFFFFFFFFull) throw std::invalid_argument("upload: upper section too large");
+  const uint64_t n = v.count;
+  const uint64_t M0 = 2 * v.M;
+  if (n > 0) {
+    if (v.vectors == nullptr || v.level0 == nullptr || v.labels == nullptr || v.levels == nullptr) {
+      throw std::invalid_argument("upload: null array");
+    }
+    if (v.entry_point >= n || v.max_level < 0 || v.levels[v.entry_point] != v.max_level) {
+      throw std::invalid_argument("upload: invalid entry point / max level");
+    }
+  }
+  // Graph checks that keep every device access in bounds, and the duplicate scan.
+  std::vector<uint32_t> off(n);
+  uint64_t pos = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    if (v.levels[i] > v.max_level) throw std::invalid_argument("upload: node level exceeds max level");
+    off[i] = static_cast<uint32_t>(pos);
+    pos += static_cast<uint64_t>(v.levels[i]) * (1 + v.M);
+  }
+  if (pos != v.upper_used) throw std::invalid_argument("upload: upper size does not match levels");
+  if (pos > 0 && v.upper == nullptr) throw std::invalid_argument("upload: null upper array");
+  uint32_t has_dups = 0;
+  std::vector<uint32_t> seen;
+  for (uint64_t i = 0; i < n; ++i) {
+    for (int L = 0; L <= v.levels[i]; ++L) {
+      const uint32_t* l = L == 0 ? v.level0 + i * (1 + M0) : v.upper + off[i] + static_cast<uint64_t>(L - 1) * (1 + v.M);
+      const uint64_t cap = L == 0 ? M0 : v.M;
+      if (l[0] > cap) throw std::invalid_argument("upload: neighbor count exceeds maximum degree");
+      for (uint32_t j = 0; j < l[0]; ++j) {
+        const uint32_t nb = l[1 + j];
+        if (nb >= n || v.levels[nb] < L) throw std::invalid_argument("upload: neighbor id out of range");
+      }
+      if (L == 0 && !has_dups) {
+        seen.assign(l + 1, l + 1 + l[0]);
+        std::sort(seen.begin(), seen.end());
+        if (std::adjacent_find(seen.begin(), seen.end()) != seen.end()) has_dups = 1;
+      }
+    }
+  }
+
+  const uint64_t words = (n + 31) / 32;
+  std::vector<uint32_t> del_bits(words, 0u);
+  h.deleted_host.assign(n, 0);
+  h.num_deleted = 0;
+  if (v.deleted != nullptr) {
+    for (uint64_t i = 0; i < n; ++i) {
+      if (v.deleted[i] != 0) {
+        del_bits[i >> 5] |= 1u << (i & 31);
+        h.deleted_host[i] = 1;
+        ++h.num_deleted;
+      }
+    }
+  }
+  auto put = [&](DevBuf& b, const void* src, size_t bytes) {
+    void* d = b.need(bytes);
+    if (bytes > 0) check(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice), "hipMemcpy upload");
+    return d;
+  };
+  h.view.vecs = static_cast<const float*>(put(h.vecs, v.vectors, n * v.padded_dim * sizeof(float)));
+  h.view.l0 = static_cast<const uint32_t*>(put(h.l0, v.level0, n * (1 + M0) * sizeof(uint32_t)));
+  h.view.upper = static_cast<const uint32_t*>(put(h.upper, v.upper, v.upper_used * sizeof(uint32_t)));
+  h.view.upper_off = static_cast<const uint32_t*>(put(h.upper_off, off.data(), n * sizeof(uint32_t)));
+  h.view.labels = static_cast<const uint64_t*>(put(h.labels, v.labels, n * sizeof(uint64_t)));
+  put(h.deleted, del_bits.data(), words * sizeof(uint32_t));
+  h.view.count = static_cast<uint32_t>(n);
+  h.view.padded = static_cast<uint32_t>(v.padded_dim);
+  h.view.M = static_cast<uint32_t>(v.M);
+  h.view.M0 = static_cast<uint32_t>(M0);
+  h.view.entry = v.entry_point;
+  h.view.max_level = v.max_level;
+  h.view.metric = v.metric;
+  h.view.has_dups = has_dups;
+  h.dim = v.dim;
+  h.ef_search = v.ef_search == 0 ? 10 : v.ef_search;
+  h.upper_used = v.upper_used;
+}
+
+/// Visited-hash size: about 48 slots per unit of ef, within [2^11, 2^14].
+uint32_t auto_hash_log2(uint32_t ef) {
+  const uint32_t l = next_log2(static_cast<uint64_t>(ef) * 48);
+  return std::min<uint32_t>(14, std::max<uint32_t>(11, l));
+}
+
+void ensure_spill(hnswgpu_index& h) {
+  const uint64_t n = h.view.count;
+  if (h.spill_alloc_count == n && h.spill_bits.p != nullptr) return;
+  const uint64_t words = (n + 31) / 32;
+  h.spill_bits.need(static_cast<size_t>(h.spill_waves) * words * sizeof(uint32_t));
+  h.spill_heaps.need(static_cast<size_t>(h.spill_waves) * (2 * n + 4) * sizeof(uint64_t));
+  h.spill_alloc_count = n;
+}
+
+/// Device-side search on device buffers (caller holds h.mu).
+void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
+                          const uint64_t* d_fw, uint64_t fsize, int by_id, hnswgpu_neighbor* d_out,
+                          uint32_t* d_counts, hipStream_t st) {
+  const uint64_t ef_eff = std::max<uint64_t>(ef == 0 ? h.ef_search : ef, k);
+  if (ef_eff > 0xFFFFFFFFull || k > 0xFFFFFFFFull || nq > 0xFFFFFFFFull) {
+    throw std::invalid_argument("search_batch: size out of range");
+  }
+  const DevIndexView& ix = h.view;
+  auto* counters = static_cast<uint32_t*>(h.counters.need(4 * sizeof(uint32_t)));
+  SearchScratch s{};
+  s.counters = counters;
+  s.overflow = static_cast<uint32_t*>(h.overflow.need(nq * sizeof(uint32_t)));
+  s.qprep = static_cast<float*>(h.qprep.need(nq * ix.padded * sizeof(float)));
+  ensure_spill(h);
+  s.spill_words = static_cast<uint32_t*>(h.spill_bits.p);
+  s.spill_heaps = static_cast<uint64_t*>(h.spill_heaps.p);
+  s.spill_heap_stride = 2ULL * ix.count + 4;
+  s.spill_waves = h.spill_waves;
+  s.spill_bitmap_words = (ix.count + 31) / 32;
+
+  SearchRequest r{};
+  r.queries = d_qs;
+  r.dim = static_cast<uint32_t>(h.dim);
+  r.nq = static_cast<uint32_t>(nq);
+  r.k = static_cast<uint32_t>(k);
+  r.ef = static_cast<uint32_t>(ef_eff);
+  r.out = reinterpret_cast<OutRow*>(d_out);
+  r.counts = d_counts;
+  r.stats = h.collect_stats ? static_cast<uint32_t*>(h.stats.need(nq * 3 * sizeof(uint32_t))) : nullptr;
+  h.last_nq = nq;
+  r.force_spill = h.force_spill;
+  r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(r.ef);
+  const size_t per_wave = (static_cast<size_t>(1u << r.hash_log2) + 192u + ix.padded) * 4u;
+  r.waves_per_block = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / per_wave)));
+  r.blocks = static_cast<uint32_t>(std::min<uint64_t>((nq + r.waves_per_block - 1) / r.waves_per_block, 8192));
+
+  const bool filtered = d_fw != nullptr || h.num_deleted > 0;
+  r.elig = nullptr;
+  if (filtered) {
+    auto* e = static_cast<uint32_t*>(h.elig.need(((ix.count + 31) / 32) * sizeof(uint32_t)));
+    check(hnswamd::gpu::launch_eligibility(ix, static_cast<const uint32_t*>(h.deleted.p), d_fw, fsize, by_id, e, st),
+          "eligibility kernel");
+    r.elig = e;
+  }
+  check(hnswamd::gpu::launch_prepare(ix, r, s, st), "prepare kernel");
+  hipEvent_t b = nullptr, e = nullptr;
+  if (h.kernel_timing) {
+    if (h.ev_begin == nullptr) check(hipEventCreate(&h.ev_begin), "hipEventCreate");
+    if (h.ev_end == nullptr) check(hipEventCreate(&h.ev_end), "hipEventCreate");
+    b = h.ev_begin;
+    e = h.ev_end;
+  }
+  check(hnswamd::gpu::launch_search(ix, r, s, st, b, e), "search kernel");
+  h.timing_pending = b != nullptr;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return HNSWGPU_OK;
+  } catch (...) {
+    return hnswamd::status_from_current_exception();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int hnswgpu_upload(const hnswgpu_index_view* view, hnswgpu_index** out) {
+  return guarded([&] {
+    if (view == nullptr || out == nullptr) throw std::invalid_argument("upload: null pointer");
+    auto* h = new hnswgpu_index();
+    try {
+      check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+      upload_view(*h, *view);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int hnswgpu_free(hnswgpu_index* h) {
+  return guarded([&] { delete h; });
+}
+
+int hnswgpu_set_deleted(hnswgpu_index* h, uint32_t id, uint8_t deleted) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("set_deleted: null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (id >= h->view.count) throw std::out_of_range("set_deleted: id out of range");
+    const uint8_t want = deleted != 0 ? 1 : 0;
+    if (h->deleted_host[id] == want) return;
+    h->deleted_host[id] = want;
+    h->num_deleted += want ? 1 : static_cast<uint64_t>(-1);
+    const uint32_t word_idx = id >> 5;
+    uint32_t word = 0;
+    for (uint32_t b = 0; b < 32; ++b) {
+      const uint64_t j = static_cast<uint64_t>(word_idx) * 32 + b;
+      if (j < h->view.count && h->deleted_host[j]) word |= 1u << b;
+    }
+    auto* d = static_cast<uint32_t*>(h->deleted.p) + word_idx;
+    check(hipMemcpyAsync(d, &word, sizeof(word), hipMemcpyHostToDevice, h->stream), "hipMemcpy deleted");
+    check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+  });
+}
+
+int hnswgpu_set_ef(hnswgpu_index* h, uint64_t ef) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("set_ef: null handle");
+    if (ef == 0) throw std::invalid_argument("set_ef: ef must be > 0");
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->ef_search = ef;
+  });
+}
+
+}  // extern "C"
+
+namespace hnswamd {
+
+/// Host-buffer search used by the C ABI and the C++ facade. Filter words are
+/// label-indexed (a BitsetFilter) or, when by_id, indexed by internal id
+/// (any other Filter, evaluated on the host).
+void device_search_host(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,
+                        const uint64_t* filter_words, uint64_t filter_size, int by_id, hnswgpu_neighbor* out,
+                        uint32_t* counts) {
+    if (h == nullptr) throw std::invalid_argument("search_batch: null handle");
+    if (k == 0) throw std::invalid_argument("search_batch: k must be > 0");
+    if (nq == 0) return;
+    if (qs == nullptr || out == nullptr) throw std::invalid_argument("search_batch: null pointer");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->view.count == 0) {
+      for (uint64_t i = 0; i < nq * k; ++i) out[i] = hnswgpu_neighbor{~0ULL, __builtin_inff(), 0u};
+      if (counts != nullptr) std::fill(counts, counts + nq, 0u);
+      return;
+    }
+    hipStream_t st = h->stream;
+    auto* d_qs = static_cast<float*>(h->host_qs.need(nq * h->dim * sizeof(float)));
+    auto* d_out = static_cast<hnswgpu_neighbor*>(h->host_out.need(nq * k * sizeof(hnswgpu_neighbor)));
+    auto* d_counts = static_cast<uint32_t*>(h->host_counts.need(nq * sizeof(uint32_t)));
+    check(hipMemcpyAsync(d_qs, qs, nq * h->dim * sizeof(float), hipMemcpyHostToDevice, st), "hipMemcpy queries");
+    const uint64_t* d_fw = nullptr;
+    if (filter_words != nullptr) {
+      const uint64_t fbits = by_id ? h->view.count : filter_size;
+      const size_t fbytes = ((fbits + 63) / 64) * sizeof(uint64_t);
+      auto* f = static_cast<uint64_t*>(h->host_filter.need(fbytes));
+      if (fbytes > 0) check(hipMemcpyAsync(f, filter_words, fbytes, hipMemcpyHostToDevice, st), "hipMemcpy filter");
+      d_fw = f;
+    }
+    search_device_locked(*h, d_qs, nq, k, ef, d_fw, filter_size, by_id, d_out, d_counts, st);
+    check(hipMemcpyAsync(out, d_out, nq * k * sizeof(hnswgpu_neighbor), hipMemcpyDeviceToHost, st),
+          "hipMemcpy results");
+    if (counts != nullptr) {
+      check(hipMemcpyAsync(counts, d_counts, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "hipMemcpy counts");
+    }
+    uint32_t ctr[4] = {};
+    check(hipMemcpyAsync(ctr, h->counters.p, sizeof(ctr), hipMemcpyDeviceToHost, st), "hipMemcpy counters");
+    check(hipStreamSynchronize(st), "search_batch");
+    h->last_overflow = ctr[1];
+}
+
+}  // namespace hnswamd
+
+extern "C" {
+
+int hnswgpu_search_batch(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,
+                         const uint64_t* filter_words, uint64_t filter_size, hnswgpu_neighbor* out,
+                         uint32_t* counts, uint64_t /*num_threads*/) {
+  return guarded([&] {
+    hnswamd::device_search_host(h, qs, nq, k, ef, filter_words, filter_size, 0, out, counts);
+  });
+}
+
+int hnswgpu_search_batch_device(hnswgpu_index* h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
+                                const uint64_t* d_filter_words, uint64_t filter_size,
+                                hnswgpu_neighbor* d_out, uint32_t* d_counts, void* stream) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("search_batch: null handle");
+    if (k == 0) throw std::invalid_argument("search_batch: k must be > 0");
+    if (nq == 0) return;
+    if (d_qs == nullptr || d_out == nullptr) throw std::invalid_argument("search_batch: null pointer");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->view.count == 0) throw std::invalid_argument("search_batch: empty index on the device path");
+    search_device_locked(*h, d_qs, nq, k, ef, d_filter_words, filter_size, 0, d_out, d_counts,
+                         static_cast<hipStream_t>(stream));
+  });
+}
+
+int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
+  return guarded([&] {
+    if (h == nullptr || name == nullptr) throw std::invalid_argument("set_option: null pointer");
+    std::lock_guard<std::mutex> lk(h->mu);
+    const std::string n(name);
+    if (n == "hash_log2") {
+      if (value != 0 && (value < 8 || value > 15)) throw std::invalid_argument("hash_log2 must be 0 or 8..15");
+      h->hash_log2 = static_cast<uint32_t>(value);
+    } else if (n == "force_spill") {
+      h->force_spill = value != 0 ? 1 : 0;
+    } else if (n == "collect_stats") {
+      h->collect_stats = value != 0 ? 1 : 0;
+    } else if (n == "kernel_timing") {
+      h->kernel_timing = value != 0 ? 1 : 0;
+    } else if (n == "spill_waves") {
+      if (value < 1 || value > 4096) throw std::invalid_argument("spill_waves must be 1..4096");
+      h->spill_waves = static_cast<uint32_t>(value);
+      h->spill_bits.release();
+      h->spill_heaps.release();
+      h->spill_alloc_count = 0;
+    } else {
+      throw std::invalid_argument("unknown option " + n);
+    }
+  });
+}
+
+int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
+  return guarded([&] {
+    if (h == nullptr || name == nullptr || value == nullptr) throw std::invalid_argument("get_option: null pointer");
+    std::lock_guard<std::mutex> lk(h->mu);
+    const std::string n(name);
+    if (n == "hash_log2") {
+      *value = h->hash_log2;
+    } else if (n == "force_spill") {
+      *value = h->force_spill;
+    } else if (n == "spill_waves") {
+      *value = h->spill_waves;
+    } else if (n == "last_overflow") {
+      *value = h->last_overflow;
+    } else if (n == "collect_stats") {
+      *value = h->collect_stats;
+    } else if (n == "kernel_timing") {
+      *value = h->kernel_timing;
+    } else if (n == "last_kernel_ns") {
+      // Fast-kernel time of the last search, from events recorded on its stream.
+      if (!h->timing_pending) throw std::invalid_argument("last_kernel_ns: enable kernel_timing and search first");
+      check(hipEventSynchronize(h->ev_end), "hipEventSynchronize");
+      float ms = 0.0f;
+      check(hipEventElapsedTime(&ms, h->ev_begin, h->ev_end), "hipEventElapsedTime");
+      *value = static_cast<int64_t>(static_cast<double>(ms) * 1e6);
+    } else if (n == "has_dups") {
+      *value = h->view.has_dups;
+    } else if (n == "num_deleted") {
+      *value = static_cast<int64_t>(h->num_deleted);
+    } else {
+      throw std::invalid_argument("unknown option " + n);
+    }
+  });
+}
+
+}  // extern "C"
+
+extern "C" int hnswgpu_read_stats(hnswgpu_index* h, uint32_t* out, uint64_t nq) {
+  return guarded([&] {
+    if (h == nullptr || out == nullptr) throw std::invalid_argument("read_stats: null pointer");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->collect_stats || h->stats.p == nullptr) throw std::invalid_argument("read_stats: collect_stats is off");
+    if (nq > h->last_nq) throw std::invalid_argument("read_stats: more queries than the last search had");
+    check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    check(hipMemcpy(out, h->stats.p, nq * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy stats");
+  });
+}

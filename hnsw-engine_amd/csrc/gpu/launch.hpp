@@ -1,0 +1,80 @@
+#pragma once
+
+// Internal interface between the C-ABI layer and the HIP kernels.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hnswamd {
+namespace gpu {
+
+/// The resident index in HBM (reference layout; rows padded to 16 floats).
+struct DevIndexView {
+  const float* vecs;         // count x padded fp32
+  const uint32_t* l0;        // count x (1 + M0)
+  const uint32_t* upper;     // upper_used
+  const uint32_t* upper_off; // count, in u32 units
+  const uint64_t* labels;    // count
+  uint32_t count;
+  uint32_t padded;
+  uint32_t M;
+  uint32_t M0;
+  uint32_t entry;
+  int32_t max_level;
+  int32_t metric;            // 0 = L2, 1 = inner product, 2 = cosine
+  uint32_t has_dups;         // some adjacency list repeats an id
+};
+
+struct OutRow {
+  uint64_t label;
+  float distance;
+  uint32_t pad;
+};
+
+/// Scratch the kernels share: [0] query ticket, [1] overflow count,
+/// [2] spill ticket, [3] spare.
+struct SearchScratch {
+  uint32_t* counters;        // 4 words
+  uint32_t* overflow;        // capacity >= nq
+  float* qprep;              // nq x padded
+  uint32_t* spill_words;     // spill_waves x visited-bitmap words
+  uint64_t* spill_heaps;     // spill_waves x (count + 1 + ef + 1) keys
+  uint64_t spill_heap_stride;
+  uint32_t spill_waves;
+  uint32_t spill_bitmap_words;
+};
+
+struct SearchRequest {
+  const float* queries;      // nq x dim (device)
+  uint32_t dim;
+  uint32_t nq;
+  uint32_t k;
+  uint32_t ef;               // already clamped: max(ef or ef_search, k)
+  const uint32_t* elig;      // id bitmap, 1 = may be returned; null = unfiltered
+  OutRow* out;               // nq x k (device)
+  uint32_t* counts;          // nq result counts (device) or null
+  uint32_t* stats;           // nq x 3 work counts {distances, layer-0 lists, upper lists} or null
+  uint32_t hash_log2;        // visited-hash slots per wave = 1 << hash_log2
+  uint32_t waves_per_block;
+  uint32_t blocks;
+  int force_spill;           // route every query through the general kernel
+};
+
+/// Prepare queries (copy into padded rows; unit-normalise for cosine).
+hipError_t launch_prepare(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st);
+/// Reset counters, run the fast kernel, then the general kernel on overflow.
+/// When both events are given they bracket the fast kernel on `st`.
+hipError_t launch_search(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st,
+                         hipEvent_t fast_begin = nullptr, hipEvent_t fast_end = nullptr);
+/// Eligibility bitmap: 1 iff not deleted and (no filter, or the filter allows
+/// the node: filter bits indexed by label, or by internal id when by_id).
+hipError_t launch_eligibility(const DevIndexView& ix, const uint32_t* deleted_bits, const uint64_t* filter_words,
+                              uint64_t filter_size, int by_id, uint32_t* elig, hipStream_t st);
+
+/// Largest ef the fast kernel holds in registers.
+constexpr uint32_t kFastMaxEf = 512;
+/// Largest M0 (layer-0 degree) the kernels load in one wave.
+constexpr uint32_t kMaxDegree0 = 64;
+
+}  // namespace gpu
+}  // namespace hnswamd

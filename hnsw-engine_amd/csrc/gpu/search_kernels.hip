@@ -1,0 +1,554 @@
+// Graph search on the GPU: greedy descent through the upper layers, then
+// Alg. 2 (search_layer) on layer 0, one wavefront per query.
+//
+// The semantics are the reference's (src/search_layer.cpp:29-171):
+//   * greedy: per pass, the first strict minimum in list order replaces the
+//     current node if it is strictly closer; passes repeat until none does;
+//   * layer 0: pop the (distance, id)-smallest candidate; stop when |R| >= ef
+//     and its distance is strictly above the worst result; visit neighbours in
+//     list order, admitting one iff |R| < ef or its distance is strictly below
+//     the worst result; admitted nodes join C and, when eligible, R, which keeps
+//     the ef (distance, id)-smallest;
+//   * output: R in (distance, id) order, labels looked up, padded with
+//     {~0, +inf}.
+// Distances are fp32 in the engine's fixed summation order (see
+// host/distance.hpp), computed by 4-lane groups that each hold 16 partial sums,
+// so they are bit-identical to the host and to the reference's AVX-512 kernels.
+//
+// Fast kernel (one wavefront per query, persistent over an atomic ticket):
+//   * R (capacity ef <= 64 * ER) and C (capacity 64 * EC) are sorted arrays of
+//     (distance, id) keys held in registers across the wave;
+//   * the visited set is an open-addressing hash in LDS;
+//   * C keeps its 64 * EC smallest entries. The smallest entry it ever dropped
+//     is remembered: the search is exact as long as that entry is never the
+//     true minimum of C at a point where the exact algorithm would go on. If it
+//     is, or if the visited hash reaches 3/4 load, the query is queued for the
+//     general kernel.
+// General kernel: the same algorithm with a visited bitmap and two binary heaps
+// in HBM (unbounded C, any ef), used for queued queries and ef > 512.
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "device_common.hpp"
+#include "launch.hpp"
+#include "quad_distance.hpp"
+
+namespace hnswamd {
+namespace gpu {
+
+namespace {
+
+constexpr int kWavesPerBlock = 4;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ __forceinline__ bool elig_bit(const uint32_t* elig, uint32_t id) {
+  return (elig[id >> 5] >> (id & 31)) & 1u;
+}
+
+struct WaveLds {
+  uint32_t* hash;  // visited hash slots
+  uint32_t* nid;   // 64 compacted neighbour ids
+  uint32_t* nd;    // 64 ordered distances
+  uint32_t* nel;   // 64 eligibility flags
+  float* q;        // the prepared query (padded)
+};
+
+/// Ordered distances of the rows w.nid[0..U) into w.nd[0..U), 16 rows per step.
+template <int PC>
+__device__ __forceinline__ void row_dists(const DevIndexView& ix, const WaveLds& w, uint32_t U) {
+  const int lane = lane_id();
+  const bool l2 = ix.metric == 0;
+  for (uint32_t base = 0; base < U; base += 16) {
+    const uint32_t r = base + static_cast<uint32_t>(lane >> 2);
+    const bool active = r < U;
+    const uint32_t id = active ? w.nid[r] : 0u;
+    const float* row = ix.vecs + static_cast<size_t>(id) * ix.padded;
+    float d = quad_raw<PC>(l2, row, w.q, ix.padded >> 4, false, active);
+    if (!l2) d = 1.0f - d;
+    if (active && (lane & 3) == 0) w.nd[r] = ord_of(d);
+  }
+}
+
+__device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    const uint64_t o = from_lane64(v, lane_id() ^ s);
+    v = o < v ? o : v;
+  }
+  return uniform64(v);
+}
+
+/// Insert id into the visited hash; true iff it was not there.
+__device__ __forceinline__ bool hash_visit(uint32_t* tab, uint32_t log2h, uint32_t id) {
+  const uint32_t mask = (1u << log2h) - 1u;
+  uint32_t h = (id * 0x9E3779B1u) >> (32u - log2h);
+  for (;;) {
+    const uint32_t old = atomicCAS(&tab[h], kEmptySlot, id);
+    if (old == kEmptySlot) return true;
+    if (old == id) return false;
+    h = (h + 1u) & mask;
+  }
+}
+
+/// Greedy descent from `cur` through layers max_level..1.
+/// Per-query work counts (distances, layer-0 list reads, upper-layer list reads).
+struct Work {
+  uint32_t dist = 0, exp0 = 0, expu = 0;
+};
+
+template <int PC>
+__device__ uint64_t descend(const DevIndexView& ix, const WaveLds& w, uint64_t cur, Work& work) {
+  const int lane = lane_id();
+  for (int L = ix.max_level; L > 0; --L) {
+    for (;;) {
+      const uint32_t cid = key_id(cur);
+      const uint32_t* l = ix.upper + ix.upper_off[cid] + static_cast<size_t>(L - 1) * (1u + ix.M);
+      const uint32_t cnt = uniform(l[0]);
+      work.dist += cnt;
+      ++work.expu;
+      wave_sync();
+      if (static_cast<uint32_t>(lane) < cnt) w.nid[lane] = l[1 + lane];
+      wave_sync();
+      row_dists<PC>(ix, w, cnt);
+      wave_sync();
+      const uint64_t mine = static_cast<uint32_t>(lane) < cnt ? key_of(w.nd[lane], static_cast<uint32_t>(lane))
+                                                              : kKeyInf;
+      const uint64_t best = wave_min64(mine);
+      if (best == kKeyInf || key_ord(best) >= key_ord(cur)) break;
+      cur = key_of(key_ord(best), uniform(w.nid[key_id(best)]));
+    }
+  }
+  return cur;
+}
+
+/// Load the prepared query into LDS and compute the entry point distance.
+template <int PC>
+__device__ uint64_t start_query(const DevIndexView& ix, const SearchScratch& s, const WaveLds& w, uint32_t qi,
+                                Work& work) {
+  const int lane = lane_id();
+  const float* src = s.qprep + static_cast<size_t>(qi) * ix.padded;
+  wave_sync();
+  for (uint32_t i = static_cast<uint32_t>(lane); i < ix.padded; i += 64) w.q[i] = src[i];
+  if (lane == 0) w.nid[0] = ix.entry;
+  wave_sync();
+  row_dists<PC>(ix, w, 1);
+  wave_sync();
+  const uint64_t cur = key_of(uniform(w.nd[0]), ix.entry);
+  work.dist += 1;
+  return descend<PC>(ix, w, cur, work);
+}
+
+/// Neighbours of `cid` on layer 0 that are new to the visited set, in list
+/// order: ids into w.nid, eligibility into w.nel. Returns their number; the
+/// visit itself is done by `visit(lane_has_id, id)`.
+template <typename Visit>
+__device__ __forceinline__ uint32_t fresh_neighbours(const DevIndexView& ix, const WaveLds& w, uint32_t cid,
+                                                     const uint32_t* elig, Visit visit) {
+  const int lane = lane_id();
+  const uint32_t* l = ix.l0 + static_cast<size_t>(cid) * (1u + ix.M0);
+  const uint32_t cnt = uniform(l[0]);
+  bool has = static_cast<uint32_t>(lane) < cnt;
+  const uint32_t nb = has ? l[1 + lane] : 0u;
+  if (ix.has_dups) {
+    // A repeated id counts as visited at its later positions.
+    for (uint32_t i = 0; i + 1 < cnt; ++i) {
+      const uint32_t vi = rdlane(nb, static_cast<int>(i));
+      if (has && static_cast<uint32_t>(lane) > i && nb == vi) has = false;
+    }
+  }
+  const bool fresh = has && visit(nb);
+  const uint64_t fm = ballot(fresh);
+  wave_sync();
+  if (fresh) {
+    const uint32_t p = lanes_below(fm);
+    w.nid[p] = nb;
+    if (elig != nullptr) w.nel[p] = elig_bit(elig, nb) ? 1u : 0u;
+  }
+  wave_sync();
+  return static_cast<uint32_t>(__popcll(fm));
+}
+
+__device__ __forceinline__ void write_pad(OutRow* row, uint32_t from, uint32_t k) {
+  for (uint32_t j = from + static_cast<uint32_t>(lane_id()); j < k; j += 64) {
+    row[j] = OutRow{~0ULL, __uint_as_float(0x7F800000u), 0u};
+  }
+}
+
+template <int ER, int EC, int PC>
+__device__ bool fast_query(const DevIndexView& ix, const SearchRequest& r, const SearchScratch& s,
+                           const WaveLds& w, uint32_t qi) {
+  const int lane = lane_id();
+  const uint32_t log2h = r.hash_log2;
+  const uint32_t H = 1u << log2h;
+  const uint32_t ccap = 64u * EC;
+  const uint32_t ef = r.ef;
+  const bool filtered = r.elig != nullptr;
+
+  for (uint32_t i = static_cast<uint32_t>(lane); i < H; i += 64) w.hash[i] = kEmptySlot;
+  Work work;
+  const uint64_t ep = start_query<PC>(ix, s, w, qi, work);
+
+  WaveSorted<ER> R;
+  WaveSorted<EC> C;
+  R.clear();
+  C.clear();
+  uint32_t sR = 0, sC = 0, tau = kOrdInf, nvis = 1;
+  uint64_t drop_min = kKeyInf;
+  wave_sync();
+  if (lane == 0) hash_visit(w.hash, log2h, key_id(ep));
+  wave_sync();
+  C.insert(ep, ccap);
+  sC = 1;
+  if (!filtered || elig_bit(r.elig, key_id(ep))) {
+    R.insert(ep, ef);
+    sR = 1;
+    tau = key_ord(ep);
+  }
+
+  for (;;) {
+    const uint64_t c = C.at(0);
+    if (drop_min < c) {
+      if (sR >= ef && key_ord(drop_min) > tau) break;
+      return false;
+    }
+    if (sC == 0) break;
+    if (sR >= ef && key_ord(c) > tau) break;
+    C.pop_front();
+    --sC;
+    if (nvis + ix.M0 > (H >> 2) * 3) return false;
+
+    const uint32_t U = fresh_neighbours(ix, w, key_id(c), r.elig,
+                                        [&](uint32_t id) { return hash_visit(w.hash, log2h, id); });
+    nvis += U;
+    work.dist += U;
+    ++work.exp0;
+    row_dists<PC>(ix, w, U);
+    wave_sync();
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t od = uniform(w.nd[u]);
+      if (sR < ef || od < tau) {
+        const uint64_t key = key_of(od, uniform(w.nid[u]));
+        const uint64_t dropped = C.insert(key, ccap);
+        if (dropped == kKeyInf) {
+          ++sC;
+        } else if (dropped < drop_min) {
+          drop_min = dropped;
+        }
+        if (!filtered || uniform(w.nel[u]) != 0u) {
+          R.insert(key, ef);
+          if (sR < ef) ++sR;
+          tau = key_ord(R.at(sR - 1));
+        }
+      }
+    }
+  }
+
+  const uint32_t kout = sR < r.k ? sR : r.k;
+  OutRow* row = r.out + static_cast<size_t>(qi) * r.k;
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    const uint32_t idx = static_cast<uint32_t>(e * 64 + lane);
+    if (idx < kout) {
+      const uint64_t key = R.v[e];
+      row[idx] = OutRow{ix.labels[key_id(key)], float_of(key_ord(key)), 0u};
+    }
+  }
+  write_pad(row, kout, r.k);
+  if (r.counts != nullptr && lane == 0) r.counts[qi] = kout;
+  if (r.stats != nullptr && lane == 0) {
+    r.stats[3 * static_cast<size_t>(qi)] = work.dist;
+    r.stats[3 * static_cast<size_t>(qi) + 1] = work.exp0;
+    r.stats[3 * static_cast<size_t>(qi) + 2] = work.expu;
+  }
+  return true;
+}
+
+template <int ER, int EC, int PC>
+__global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView ix, SearchRequest r,
+                                                                   SearchScratch s) {
+  extern __shared__ uint32_t lds[];
+  const int lane = lane_id();
+  const uint32_t H = 1u << r.hash_log2;
+  const uint32_t per_wave = H + 192u + ix.padded;
+  uint32_t* base = lds + static_cast<size_t>(threadIdx.x >> 6) * per_wave;
+  const WaveLds w{base, base + H, base + H + 64, base + H + 128, reinterpret_cast<float*>(base + H + 192)};
+  for (;;) {
+    uint32_t qi = 0;
+    if (lane == 0) qi = atomicAdd(&s.counters[0], 1u);
+    qi = uniform(qi);
+    if (qi >= r.nq) break;
+    bool ok = false;
+    if (!r.force_spill) ok = fast_query<ER, EC, PC>(ix, r, s, w, qi);
+    if (!ok && lane == 0) {
+      const uint32_t slot = atomicAdd(&s.counters[1], 1u);
+      s.overflow[slot] = qi;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// General kernel: binary heaps and a visited bitmap in HBM; lane 0 does the
+// heap work while the wave computes distances.
+// ---------------------------------------------------------------------------
+__device__ void heap_push(uint64_t* h, uint32_t& n, uint64_t x, bool max_heap) {
+  uint32_t i = n++;
+  while (i > 0) {
+    const uint32_t p = (i - 1) >> 1;
+    const bool up = max_heap ? (h[p] < x) : (x < h[p]);
+    if (!up) break;
+    h[i] = h[p];
+    i = p;
+  }
+  h[i] = x;
+}
+
+__device__ uint64_t heap_pop(uint64_t* h, uint32_t& n, bool max_heap) {
+  const uint64_t top = h[0];
+  const uint64_t x = h[--n];
+  uint32_t i = 0;
+  for (;;) {
+    const uint32_t a = 2 * i + 1;
+    if (a >= n) break;
+    uint32_t c = a;
+    if (a + 1 < n) {
+      const bool right = max_heap ? (h[a] < h[a + 1]) : (h[a + 1] < h[a]);
+      if (right) c = a + 1;
+    }
+    const bool down = max_heap ? (x < h[c]) : (h[c] < x);
+    if (!down) break;
+    h[i] = h[c];
+    i = c;
+  }
+  if (n > 0) h[i] = x;
+  return top;
+}
+
+__global__ void __launch_bounds__(kWavesPerBlock * 64) general_search(DevIndexView ix, SearchRequest r,
+                                                                      SearchScratch s) {
+  extern __shared__ uint32_t lds[];
+  const int lane = lane_id();
+  const uint32_t wave_slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave_slot >= s.spill_waves) return;
+  const uint32_t per_wave = 192u + ix.padded;
+  uint32_t* base = lds + static_cast<size_t>(threadIdx.x >> 6) * per_wave;
+  const WaveLds w{nullptr, base, base + 64, base + 128, reinterpret_cast<float*>(base + 192)};
+  uint32_t* bits = s.spill_words + static_cast<size_t>(wave_slot) * s.spill_bitmap_words;
+  uint64_t* Cheap = s.spill_heaps + static_cast<size_t>(wave_slot) * s.spill_heap_stride;
+  uint64_t* Rheap = Cheap + (ix.count + 1u);
+  const bool filtered = r.elig != nullptr;
+  const uint32_t n_over = uniform(*reinterpret_cast<volatile uint32_t*>(&s.counters[1]));
+
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&s.counters[2], 1u);
+    t = uniform(t);
+    if (t >= n_over) break;
+    const uint32_t qi = s.overflow[t];
+
+    for (uint32_t i = static_cast<uint32_t>(lane); i < s.spill_bitmap_words; i += 64) bits[i] = 0u;
+    __threadfence();
+    Work work;
+    const uint64_t ep = start_query<0>(ix, s, w, qi, work);
+    uint32_t nC = 0, nR = 0;
+    if (lane == 0) {
+      atomicOr(&bits[key_id(ep) >> 5], 1u << (key_id(ep) & 31));
+      heap_push(Cheap, nC, ep, false);
+      if (!filtered || elig_bit(r.elig, key_id(ep))) heap_push(Rheap, nR, ep, true);
+    }
+    for (;;) {
+      uint64_t c = kKeyInf;
+      if (lane == 0) {
+        if (nC > 0) {
+          c = Cheap[0];
+          if (nR >= r.ef && key_ord(c) > key_ord(Rheap[0])) {
+            c = kKeyInf;
+          } else {
+            heap_pop(Cheap, nC, false);
+          }
+        }
+      }
+      c = uniform64(c);
+      if (c == kKeyInf) break;
+      const uint32_t U = fresh_neighbours(ix, w, key_id(c), r.elig, [&](uint32_t id) {
+        const uint32_t old = atomicOr(&bits[id >> 5], 1u << (id & 31));
+        return (old & (1u << (id & 31))) == 0u;
+      });
+      work.dist += U;
+      ++work.exp0;
+      row_dists<0>(ix, w, U);
+      wave_sync();
+      if (lane == 0) {
+        for (uint32_t u = 0; u < U; ++u) {
+          const uint32_t od = w.nd[u];
+          if (nR < r.ef || od < key_ord(Rheap[0])) {
+            const uint64_t key = key_of(od, w.nid[u]);
+            heap_push(Cheap, nC, key, false);
+            if (!filtered || w.nel[u] != 0u) {
+              heap_push(Rheap, nR, key, true);
+              if (nR > r.ef) heap_pop(Rheap, nR, true);
+            }
+          }
+        }
+      }
+      wave_sync();
+    }
+    // Sorted results: pop the max-heap from the back into the C area.
+    if (lane == 0) {
+      const uint32_t total = nR;
+      for (uint32_t i = total; i-- > 0;) Cheap[i] = heap_pop(Rheap, nR, true);
+      nR = total;
+    }
+    __threadfence();
+    nR = uniform(nR);
+    const uint32_t kout = nR < r.k ? nR : r.k;
+    OutRow* row = r.out + static_cast<size_t>(qi) * r.k;
+    for (uint32_t j = static_cast<uint32_t>(lane); j < kout; j += 64) {
+      const uint64_t key = Cheap[j];
+      row[j] = OutRow{ix.labels[key_id(key)], float_of(key_ord(key)), 0u};
+    }
+    write_pad(row, kout, r.k);
+    if (r.counts != nullptr && lane == 0) r.counts[qi] = kout;
+    if (r.stats != nullptr && lane == 0) {
+      r.stats[3 * static_cast<size_t>(qi)] = work.dist;
+      r.stats[3 * static_cast<size_t>(qi) + 1] = work.exp0;
+      r.stats[3 * static_cast<size_t>(qi) + 2] = work.expu;
+    }
+    __threadfence();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small kernels.
+// ---------------------------------------------------------------------------
+__global__ void reset_counters(uint32_t* c) {
+  if (threadIdx.x < 4) c[threadIdx.x] = 0u;
+}
+
+/// One wavefront per query: copy into a zero-padded row, normalising for
+/// cosine with the norm summed sequentially in double (src/space.hpp:31-40).
+__global__ void __launch_bounds__(256) prepare_queries(DevIndexView ix, SearchRequest r, SearchScratch s) {
+  const int lane = lane_id();
+  const uint32_t qi = blockIdx.x * 4u + (threadIdx.x >> 6);
+  if (qi >= r.nq) return;
+  const float* src = r.queries + static_cast<size_t>(qi) * r.dim;
+  float* dst = s.qprep + static_cast<size_t>(qi) * ix.padded;
+  float inv = 1.0f;
+  if (ix.metric == 2) {
+    double ss = 0.0;
+    if (lane == 0) {
+      for (uint32_t i = 0; i < r.dim; ++i) {
+        const double x = static_cast<double>(src[i]);
+        ss = ss + x * x;
+      }
+    }
+    ss = __shfl(ss, 0);
+    inv = ss > 0.0 ? static_cast<float>(1.0 / __builtin_sqrt(ss)) : 1.0f;
+  }
+  for (uint32_t i = static_cast<uint32_t>(lane); i < ix.padded; i += 64) {
+    dst[i] = i < r.dim ? (ix.metric == 2 ? src[i] * inv : src[i]) : 0.0f;
+  }
+}
+
+__global__ void build_eligibility(uint32_t count, const uint64_t* labels, const uint32_t* deleted_bits,
+                                  const uint64_t* fw, uint64_t fsize, int by_id, uint32_t* elig) {
+  const uint32_t word = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t words = (count + 31u) >> 5;
+  if (word >= words) return;
+  uint32_t out = 0u;
+  const uint32_t del = deleted_bits != nullptr ? deleted_bits[word] : 0u;
+  for (uint32_t b = 0; b < 32; ++b) {
+    const uint32_t id = word * 32u + b;
+    if (id >= count) break;
+    bool ok = ((del >> b) & 1u) == 0u;
+    if (ok && fw != nullptr && by_id) {
+      ok = ((fw[id >> 6] >> (id & 63)) & 1ULL) != 0ULL;
+    } else if (ok && fw != nullptr) {
+      const uint64_t lab = labels[id];
+      ok = lab < fsize && ((fw[lab >> 6] >> (lab & 63)) & 1ULL) != 0ULL;
+    }
+    if (ok) out |= 1u << b;
+  }
+  elig[word] = out;
+}
+
+template <int ER, int EC, int PC>
+hipError_t run_fast(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
+  const size_t lds = static_cast<size_t>(r.waves_per_block) * ((1u << r.hash_log2) + 192u + ix.padded) * 4u;
+  static bool configured = false;
+  if (!configured) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fast_search<ER, EC, PC>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    configured = true;
+  }
+  hipLaunchKernelGGL((fast_search<ER, EC, PC>), dim3(r.blocks), dim3(r.waves_per_block * 64), lds, st, ix, r, s);
+  return hipGetLastError();
+}
+
+template <int PC>
+hipError_t pick_capacity(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
+  if (r.ef <= 64) return run_fast<1, 2, PC>(ix, r, s, st);
+  if (r.ef <= 128) return run_fast<2, 4, PC>(ix, r, s, st);
+  if (r.ef <= 256) return run_fast<4, 8, PC>(ix, r, s, st);
+  return run_fast<8, 16, PC>(ix, r, s, st);
+}
+
+}  // namespace
+
+hipError_t launch_prepare(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
+  if (r.nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(prepare_queries, dim3((r.nq + 3) / 4), dim3(256), 0, st, ix, r, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_eligibility(const DevIndexView& ix, const uint32_t* deleted_bits, const uint64_t* fw,
+                              uint64_t fsize, int by_id, uint32_t* elig, hipStream_t st) {
+  const uint32_t words = (ix.count + 31u) >> 5;
+  if (words == 0) return hipSuccess;
+  hipLaunchKernelGGL(build_eligibility, dim3((words + 255) / 256), dim3(256), 0, st, ix.count, ix.labels,
+                     deleted_bits, fw, fsize, by_id, elig);
+  return hipGetLastError();
+}
+
+hipError_t launch_search(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st,
+                         hipEvent_t fast_begin, hipEvent_t fast_end) {
+  hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, s.counters);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const bool fast = !r.force_spill && r.ef <= kFastMaxEf;
+  SearchRequest rr = r;
+  rr.force_spill = fast ? 0 : 1;
+  if (fast_begin != nullptr && fast_end != nullptr) {
+    e = hipEventRecord(fast_begin, st);
+    if (e != hipSuccess) return e;
+  }
+  if (ix.padded == 128) {
+    e = pick_capacity<8>(ix, rr, s, st);
+  } else {
+    e = pick_capacity<0>(ix, rr, s, st);
+  }
+  if (e != hipSuccess) return e;
+  if (fast_begin != nullptr && fast_end != nullptr) {
+    e = hipEventRecord(fast_end, st);
+    if (e != hipSuccess) return e;
+  }
+  const size_t lds = static_cast<size_t>(kWavesPerBlock) * (192u + ix.padded) * 4u;
+  const uint32_t blocks = (s.spill_waves + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (blocks == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(general_search, dim3(blocks), dim3(kWavesPerBlock * 64), lds, st, ix, rr, s);
+  return hipGetLastError();
+}
+
+}  // namespace gpu
+}  // namespace hnswamd

@@ -1,0 +1,297 @@
+"""Python view of the engine's C ABI (include/hnswgpu.h) over ctypes.
+
+The classes mirror the reference's Python binding (python/bindings.cpp:121-233
+of the reference): ``Index(dim, metric, M, ef_construction, max_elements,
+seed)``, ``add``, ``search`` returning ``(ids, distances)`` with -1 / +inf
+padding, ``mark_deleted``, ``save`` / ``load``; searches run on the GPU.
+``DeviceIndex`` exposes the device layer (``hnswgpu_*``) for device buffers
+and tuning. Nothing here computes: every call crosses into libhnswgpu.so, and
+importing fails loudly when the library has not been built.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libhnswgpu.so"
+
+METRICS = {"l2": 0, "L2": 0, "euclidean": 0, "ip": 1, "inner_product": 1, "dot": 1, "cosine": 2, "angular": 2}
+METRIC_NAMES = {0: "l2", 1: "ip", 2: "cosine"}
+
+NEIGHBOR = np.dtype([("label", "<u8"), ("distance", "<f4"), ("reserved", "<u4")])
+INVALID_LABEL = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+OK, EINVAL, ERUNTIME, EDEVICE, ERANGE = 0, -1, -2, -3, -4
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C hnsw-engine_amd)")
+    lib = C.CDLL(str(LIB_PATH))
+    p, u64, i32, i64 = C.c_void_p, C.c_uint64, C.c_int32, C.c_int64
+    sig = {
+        "hnswgpu_last_error": ([], C.c_char_p),
+        "hnswgpu_upload": ([p, p], C.c_int),
+        "hnswgpu_free": ([p], C.c_int),
+        "hnswgpu_set_deleted": ([p, C.c_uint32, C.c_uint8], C.c_int),
+        "hnswgpu_set_ef": ([p, u64], C.c_int),
+        "hnswgpu_search_batch": ([p, p, u64, u64, u64, p, u64, p, p, u64], C.c_int),
+        "hnswgpu_search_batch_device": ([p, p, u64, u64, u64, p, u64, p, p, p], C.c_int),
+        "hnswgpu_set_option": ([p, C.c_char_p, i64], C.c_int),
+        "hnswgpu_get_option": ([p, C.c_char_p, p], C.c_int),
+        "hnswgpu_read_stats": ([p, p, u64], C.c_int),
+        "hnsw_index_create": ([u64, i32, u64, u64, u64, u64, p], C.c_int),
+        "hnsw_index_load": ([C.c_char_p, C.c_int, p], C.c_int),
+        "hnsw_index_free": ([p], None),
+        "hnsw_index_save": ([p, C.c_char_p], C.c_int),
+        "hnsw_index_add_batch": ([p, p, p, u64, u64], C.c_int),
+        "hnsw_index_search_batch": ([p, p, u64, u64, u64, p, u64, p, u64], C.c_int),
+        "hnsw_index_search": ([p, p, u64, u64, p, u64, p, p], C.c_int),
+        "hnsw_index_mark_deleted": ([p, u64], C.c_int),
+        "hnsw_index_is_deleted": ([p, u64, p], C.c_int),
+        "hnsw_index_contains": ([p, u64, p], C.c_int),
+        "hnsw_index_set_ef": ([p, u64], C.c_int),
+        "hnsw_index_get_vector": ([p, u64, p], C.c_int),
+        "hnsw_index_info": ([p, p], C.c_int),
+        "hnsw_index_device": ([p, p], C.c_int),
+        "hnsw_brute_force_create": ([u64, i32, p], C.c_int),
+        "hnsw_brute_force_free": ([p], None),
+        "hnsw_brute_force_add": ([p, p, p, u64], C.c_int),
+        "hnsw_brute_force_search_batch": ([p, p, u64, u64, p, u64, p, u64], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    return lib
+
+
+_lib = _load()
+
+
+def _check(rc: int) -> None:
+    if rc == OK:
+        return
+    msg = (_lib.hnswgpu_last_error() or b"").decode()
+    if rc == EINVAL:
+        raise ValueError(msg)
+    if rc == ERANGE:
+        raise IndexError(msg)
+    raise RuntimeError(msg)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(x, dim, what):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    single = a.ndim == 1
+    if single:
+        a = a.reshape(1, -1)
+    if a.ndim != 2 or a.shape[1] != dim:
+        raise ValueError(f"{what}: expected shape (n, {dim}) or ({dim},), got {np.shape(x)}")
+    return a, single
+
+
+def filter_words(filter, num_labels=None):
+    """(words, size) of a label bitset from a bool mask over labels, an array of
+    allowed labels, or an existing (words, size) pair (None -> (None, 0))."""
+    if filter is None:
+        return None, 0
+    if isinstance(filter, tuple):
+        w, size = filter
+        return np.ascontiguousarray(w, dtype=np.uint64), int(size)
+    a = np.asarray(filter)
+    if a.dtype == np.bool_:
+        size = a.size
+        bits = np.zeros(max(1, (size + 63) // 64) * 64, dtype=np.uint64)
+        bits[:size] = a.reshape(-1)
+        words = np.bitwise_or.reduce(bits.reshape(-1, 64) << np.arange(64, dtype=np.uint64), axis=1)
+        return np.ascontiguousarray(words, dtype=np.uint64), size
+    labels = a.astype(np.uint64).reshape(-1)
+    size = int(labels.max()) + 1 if labels.size else 0
+    words = np.zeros(max(1, (size + 63) // 64), dtype=np.uint64)
+    np.bitwise_or.at(words, (labels >> np.uint64(6)).astype(np.int64), np.uint64(1) << (labels & np.uint64(63)))
+    return words, size
+
+
+class DeviceIndex:
+    """The device layer of an index (hnswgpu_*). Owned by its Index."""
+
+    def __init__(self, handle, owner=None):
+        self._h = handle
+        self._owner = owner
+
+    def set_option(self, name: str, value: int) -> None:
+        _check(_lib.hnswgpu_set_option(self._h, name.encode(), int(value)))
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int64()
+        _check(_lib.hnswgpu_get_option(self._h, name.encode(), C.byref(v)))
+        return v.value
+
+    def search_rows(self, queries, k, ef=0, filter=None, dim=None):
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        out = np.empty(nq * k, dtype=NEIGHBOR)
+        counts = np.empty(nq, dtype=np.uint32)
+        w, size = filter_words(filter)
+        _check(_lib.hnswgpu_search_batch(self._h, _ptr(q), nq, k, ef, _ptr(w), size, _ptr(out), _ptr(counts), 0))
+        return out.reshape(nq, k), counts
+
+    def read_stats(self, nq: int):
+        out = np.empty((nq, 3), dtype=np.uint32)
+        _check(_lib.hnswgpu_read_stats(self._h, _ptr(out), nq))
+        return out
+
+    def search_device(self, d_queries: int, nq: int, k: int, ef: int, d_out: int, stream: int = 0,
+                      d_filter: int = 0, filter_size: int = 0, d_counts: int = 0) -> None:
+        _check(_lib.hnswgpu_search_batch_device(self._h, d_queries, nq, k, ef, d_filter or None, filter_size,
+                                                d_out, d_counts or None, stream or None))
+
+
+class Index:
+    """hnsw::Index over the C ABI."""
+
+    def __init__(self, dim: int, metric: str = "l2", M: int = 16, ef_construction: int = 200,
+                 max_elements: int = 0, seed: int = 100, *, _handle=None):
+        if _handle is not None:
+            self._h = _handle
+            return
+        if metric not in METRICS:
+            raise ValueError(f"unknown metric '{metric}' (expected l2, ip or cosine)")
+        h = C.c_void_p()
+        _check(_lib.hnsw_index_create(dim, METRICS[metric], M, ef_construction, max_elements, seed, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.hnsw_index_free(h)
+            self._h = None
+
+    @staticmethod
+    def load(path, mmap: bool = False) -> "Index":
+        h = C.c_void_p()
+        _check(_lib.hnsw_index_load(os.fsencode(path), 1 if mmap else 0, C.byref(h)))
+        return Index(0, _handle=h)
+
+    def save(self, path) -> None:
+        _check(_lib.hnsw_index_save(self._h, os.fsencode(path)))
+
+    def _info(self):
+        a = np.zeros(10, dtype=np.int64)
+        _check(_lib.hnsw_index_info(self._h, _ptr(a)))
+        return a
+
+    def add(self, vectors, ids=None, num_threads: int = 0) -> None:
+        v, _ = _f32(vectors, self.dim, "vectors")
+        n = v.shape[0]
+        if ids is None:
+            labels = np.arange(len(self), len(self) + n, dtype=np.uint64)
+        else:
+            labels = np.ascontiguousarray(np.asarray(ids, dtype=np.uint64).reshape(-1))
+            if labels.size != n:
+                raise ValueError("ids: length must match the number of vectors")
+        _check(_lib.hnsw_index_add_batch(self._h, _ptr(v), _ptr(labels), n, num_threads))
+
+    def search_rows(self, queries, k: int = 10, ef=None, filter=None):
+        """Raw Neighbor rows (nq, k) with fields label/distance."""
+        if k <= 0:
+            raise ValueError("k must be > 0")
+        q, _ = _f32(queries, self.dim, "queries")
+        out = np.empty(q.shape[0] * k, dtype=NEIGHBOR)
+        w, size = filter_words(filter)
+        _check(_lib.hnsw_index_search_batch(self._h, _ptr(q), q.shape[0], k, 0 if ef is None else ef, _ptr(out),
+                                            0, _ptr(w), size))
+        return out.reshape(q.shape[0], k)
+
+    def search(self, queries, k: int = 10, ef=None, num_threads: int = 0, filter=None):
+        _, single = _f32(queries, self.dim, "queries")
+        rows = self.search_rows(queries, k, ef, filter)
+        ids = rows["label"].astype(np.int64)  # kInvalidLabel -> -1
+        d = rows["distance"].copy()
+        return (ids[0], d[0]) if single else (ids, d)
+
+    def search_one(self, query, k: int, ef: int = 0, filter=None):
+        """Index::search: up to k rows (no padding)."""
+        q, _ = _f32(query, self.dim, "query")
+        out = np.empty(k, dtype=NEIGHBOR)
+        n = C.c_uint64()
+        w, size = filter_words(filter)
+        _check(_lib.hnsw_index_search(self._h, _ptr(q), k, ef, _ptr(w), size, _ptr(out), C.byref(n)))
+        return out[: n.value]
+
+    def set_ef(self, ef: int) -> None:
+        _check(_lib.hnsw_index_set_ef(self._h, ef))
+
+    def mark_deleted(self, label: int) -> None:
+        if label < 0:
+            raise ValueError("labels are non-negative")
+        _check(_lib.hnsw_index_mark_deleted(self._h, label))
+
+    def is_deleted(self, label: int) -> bool:
+        o = C.c_int()
+        _check(_lib.hnsw_index_is_deleted(self._h, label, C.byref(o)))
+        return bool(o.value)
+
+    def __contains__(self, label: int) -> bool:
+        o = C.c_int()
+        _check(_lib.hnsw_index_contains(self._h, label, C.byref(o)))
+        return bool(o.value)
+
+    def get_vector(self, label: int):
+        out = np.empty(self.dim, dtype=np.float32)
+        _check(_lib.hnsw_index_get_vector(self._h, label, _ptr(out)))
+        return out
+
+    def device(self) -> DeviceIndex:
+        h = C.c_void_p()
+        _check(_lib.hnsw_index_device(self._h, C.byref(h)))
+        return DeviceIndex(h, self)
+
+    def __len__(self) -> int:
+        return int(self._info()[0])
+
+    dim = property(lambda self: int(self._info()[1]))
+    metric = property(lambda self: METRIC_NAMES[int(self._info()[2])])
+    M = property(lambda self: int(self._info()[3]))
+    ef_construction = property(lambda self: int(self._info()[4]))
+    max_elements = property(lambda self: int(self._info()[5]))
+    ef = property(lambda self: int(self._info()[6]), lambda self, v: self.set_ef(v))
+    max_level = property(lambda self: int(self._info()[7]))
+    read_only = property(lambda self: bool(self._info()[8]))
+    memory_usage = property(lambda self: int(self._info()[9]))
+
+
+class BruteForceIndex:
+    """hnsw::BruteForceIndex over the C ABI (exact k-NN on the GPU)."""
+
+    def __init__(self, dim: int, metric: str = "l2"):
+        h = C.c_void_p()
+        _check(_lib.hnsw_brute_force_create(dim, METRICS[metric], C.byref(h)))
+        self._h, self.dim = h, dim
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.hnsw_brute_force_free(h)
+            self._h = None
+
+    def add(self, vectors, labels=None) -> None:
+        v, _ = _f32(vectors, self.dim, "vectors")
+        lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.uint64)
+        _check(_lib.hnsw_brute_force_add(self._h, _ptr(v), _ptr(lab), v.shape[0]))
+
+    def search_rows(self, queries, k: int, filter=None):
+        q, _ = _f32(queries, self.dim, "queries")
+        out = np.empty(q.shape[0] * k, dtype=NEIGHBOR)
+        w, size = filter_words(filter)
+        _check(_lib.hnsw_brute_force_search_batch(self._h, _ptr(q), q.shape[0], k, _ptr(out), 0, _ptr(w), size))
+        return out.reshape(q.shape[0], k)

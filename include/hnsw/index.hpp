@@ -1,0 +1,117 @@
+#pragma once
+
+// The C++ API of the engine: the reference's hnsw::Index and BruteForceIndex
+// (include/hnsw/index.hpp:14-118 of the reference) with the same signatures and
+// semantics, searching on the GPU. Indexes are built on the host (the
+// reference's algorithm) and stored in the reference's file format, so a file
+// written by either library loads in the other. The device copy is uploaded on
+// first search and kept resident; add() marks it stale, mark_deleted() and
+// set_ef() update it in place.
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "hnsw/filter.hpp"
+#include "hnsw/metric.hpp"
+
+namespace hnsw {
+
+struct Params {
+  std::size_t dim = 0;
+  Metric metric = Metric::L2;
+  std::size_t M = 16;
+  std::size_t ef_construction = 200;
+  std::size_t max_elements = 0;  ///< 0 = grow geometrically
+  uint64_t seed = 100;
+};
+
+struct Neighbor {
+  uint64_t label;
+  float distance;
+};
+
+/// Label that pads search_batch rows when fewer than k results exist.
+inline constexpr uint64_t kInvalidLabel = ~uint64_t{0};
+
+/// Thread safety: search / search_batch / accessors may run concurrently;
+/// add / add_batch / mark_deleted / set_ef are exclusive.
+class Index {
+ public:
+  explicit Index(const Params& params);
+  ~Index();
+  Index(Index&&) noexcept;
+  Index& operator=(Index&&) noexcept;
+  Index(const Index&) = delete;
+  Index& operator=(const Index&) = delete;
+
+  void add(const float* vec, uint64_t label);
+  void add_batch(const float* vecs, const uint64_t* labels, std::size_t n, std::size_t num_threads = 0);
+
+  /// k nearest neighbours sorted by increasing distance; ef = 0 uses the index
+  /// default, ef is clamped to >= k; returns min(k, #eligible) results.
+  std::vector<Neighbor> search(const float* q, std::size_t k, std::size_t ef = 0,
+                               const Filter* f = nullptr) const;
+  /// nq queries into nq*k rows, padded with {kInvalidLabel, +inf}. num_threads
+  /// is accepted for signature compatibility; the work runs on the GPU.
+  void search_batch(const float* qs, std::size_t nq, std::size_t k, std::size_t ef, Neighbor* out,
+                    std::size_t num_threads = 0, const Filter* f = nullptr) const;
+
+  void mark_deleted(uint64_t label);
+  bool is_deleted(uint64_t label) const;
+  bool contains(uint64_t label) const;
+
+  std::size_t size() const;
+  std::size_t dim() const noexcept;
+  Metric metric() const noexcept;
+  const Params& params() const noexcept;
+  std::size_t capacity() const;
+  std::size_t ef() const;
+  void set_ef(std::size_t ef);
+  int max_level() const;
+  bool read_only() const noexcept;
+  std::size_t memory_usage() const;
+  std::vector<float> get_vector(uint64_t label) const;
+
+  void save(const std::string& path) const;
+  static Index load(const std::string& path, bool mmap = false);
+
+  struct Impl;
+  /// The device-layer handle (struct hnswgpu_index from hnswgpu.h), uploading
+  /// or refreshing it first if needed.
+  void* device_handle() const;
+
+ private:
+  explicit Index(std::unique_ptr<Impl> impl);
+  std::unique_ptr<Impl> impl_;
+};
+
+/// Exact search over all vectors (GPU); the recall oracle.
+class BruteForceIndex {
+ public:
+  BruteForceIndex(std::size_t dim, Metric metric);
+  ~BruteForceIndex();
+  BruteForceIndex(BruteForceIndex&&) noexcept;
+  BruteForceIndex& operator=(BruteForceIndex&&) noexcept;
+  void add(const float* vecs, const uint64_t* labels, std::size_t n);
+  std::vector<Neighbor> search(const float* q, std::size_t k, const Filter* f = nullptr) const;
+  void search_batch(const float* qs, std::size_t nq, std::size_t k, Neighbor* out, std::size_t num_threads = 0,
+                    const Filter* f = nullptr) const;
+  std::size_t size() const noexcept { return labels_.size(); }
+  std::size_t dim() const noexcept { return dim_; }
+
+  struct Device;
+
+ private:
+  void run(const float* qs, std::size_t nq, std::size_t k, Neighbor* out, uint32_t* counts,
+           const Filter* f) const;
+  std::size_t dim_;
+  Metric metric_;
+  std::vector<float> data_;
+  std::vector<uint64_t> labels_;
+  mutable std::unique_ptr<Device> dev_;
+};
+
+}  // namespace hnsw

@@ -1,0 +1,181 @@
+/*
+ * hnswgpu.h -- the C ABI of the MI355X (gfx950) HNSW search engine.
+ *
+ * Plain pointers, sizes and opaque handles; no C++ or torch types cross it.
+ * Every function returns HNSWGPU_OK (0) or a negative status whose text is in
+ * hnswgpu_last_error() (thread-local). Status classes mirror the exception
+ * types of the reference C++ API: HNSWGPU_EINVAL <-> std::invalid_argument,
+ * HNSWGPU_ERUNTIME <-> std::runtime_error, HNSWGPU_ERANGE <-> std::out_of_range.
+ *
+ * Two layers:
+ *   hnsw_index_*  the reference's hnsw::Index surface (include/hnsw/index.hpp of
+ *                 the reference): build, load/save in the reference file format,
+ *                 soft deletes, and search/search_batch on the GPU.
+ *   hnswgpu_*     the device layer underneath: upload a loaded index to HBM once,
+ *                 then search it with host or device buffers.
+ * plus hnsw_brute_force_* for BruteForceIndex.
+ */
+#ifndef HNSWGPU_H_
+#define HNSWGPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HNSWGPU_OK 0
+#define HNSWGPU_EINVAL (-1)
+#define HNSWGPU_ERUNTIME (-2)
+#define HNSWGPU_EDEVICE (-3)
+#define HNSWGPU_ERANGE (-4)
+
+/* Metric ids as stored in the file header (docs/FORMAT.md): 0 L2, 1 inner product, 2 cosine. */
+#define HNSWGPU_METRIC_L2 0
+#define HNSWGPU_METRIC_IP 1
+#define HNSWGPU_METRIC_COSINE 2
+
+/* Same layout as the reference's hnsw::Neighbor {u64 label; f32 distance;}
+ * (sizeof 16, include/hnsw/index.hpp:23-26). Missing results are
+ * {~0 (kInvalidLabel), +inf}. */
+typedef struct hnswgpu_neighbor {
+  uint64_t label;
+  float distance;
+  uint32_t reserved;
+} hnswgpu_neighbor;
+
+/* Thread-local text of the last failure on this thread. */
+const char* hnswgpu_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Device layer                                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct hnswgpu_index hnswgpu_index;
+
+/* A loaded index in the reference's in-memory layout (src/index_impl.hpp:176-237):
+ * host pointers, borrowed for the duration of hnswgpu_upload only. */
+typedef struct hnswgpu_index_view {
+  uint64_t dim;
+  uint64_t padded_dim;      /* dim rounded up to 16 */
+  uint64_t M;               /* upper-layer degree; layer 0 has 2M */
+  uint64_t count;
+  uint64_t ef_search;       /* default ef when a search passes ef = 0 */
+  uint64_t upper_used;      /* u32 slots in `upper` */
+  int32_t metric;
+  int32_t max_level;
+  uint32_t entry_point;
+  uint32_t reserved;
+  const float* vectors;     /* count x padded_dim */
+  const uint32_t* level0;   /* count x (1 + 2M) */
+  const uint32_t* upper;    /* per node of level L >= 1: L blocks of (1 + M) */
+  const uint64_t* labels;   /* count */
+  const uint8_t* levels;    /* count */
+  const uint8_t* deleted;   /* count tombstone bytes, or NULL for none */
+} hnswgpu_index_view;
+
+/* Replaces: the arrays Index::load leaves in memory (src/persist.cpp:477-541),
+ * which the device layer uploads once to HBM and keeps resident. */
+int hnswgpu_upload(const hnswgpu_index_view* view, hnswgpu_index** out);
+
+/* Releases the device copy. */
+int hnswgpu_free(hnswgpu_index* h);
+
+/* Replaces: the tombstone write of Index::mark_deleted (src/index.cpp:308-318),
+ * addressed by internal id. */
+int hnswgpu_set_deleted(hnswgpu_index* h, uint32_t id, uint8_t deleted);
+
+/* Replaces: Index::set_ef (src/index.cpp:352-356): the ef used when a search
+ * passes ef = 0. */
+int hnswgpu_set_ef(hnswgpu_index* h, uint64_t ef);
+
+/* Replaces: Index::search_batch (include/hnsw/index.hpp:59-60,
+ * src/search_layer.cpp:180-194). Host buffers, synchronous. qs is nq x dim
+ * (unpadded), out is nq x k, padded with {~0, +inf}. The optional filter is a
+ * BitsetFilter's words over labels (bit l set = label l allowed, labels >=
+ * filter_size rejected; include/hnsw/filter.hpp:31-48). counts (nullable)
+ * receives min(k, #results) per query, the length Index::search returns.
+ * num_threads is accepted for signature parity and ignored. */
+int hnswgpu_search_batch(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,
+                         const uint64_t* filter_words, uint64_t filter_size, hnswgpu_neighbor* out,
+                         uint32_t* counts, uint64_t num_threads);
+
+/* Same search on device buffers, asynchronous on `stream` (a hipStream_t, NULL
+ * = the default stream). Calls on one handle must be ordered (same stream). */
+int hnswgpu_search_batch_device(hnswgpu_index* h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
+                                const uint64_t* d_filter_words, uint64_t filter_size,
+                                hnswgpu_neighbor* d_out, uint32_t* d_counts, void* stream);
+
+/* Tuning and test switches (no reference counterpart):
+ *   "hash_log2"     visited-hash slots per wavefront (0 = automatic)
+ *   "force_spill"   1 = every query through the general (HBM heap) kernel
+ *   "spill_waves"   wavefronts of the general kernel
+ *   "collect_stats" 1 = record per-query work counts (hnswgpu_read_stats)
+ *   "kernel_timing" 1 = bracket the fast kernel with events on its stream
+ * and read-only values of the last search:
+ *   "last_overflow"  queries the fast kernel handed to the general kernel
+ *   "last_kernel_ns" fast-kernel time (needs kernel_timing; waits for it). */
+int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value);
+int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value);
+
+/* Per-query work of the last search (collect_stats on): nq x 3 of
+ * {distances computed, layer-0 lists read, upper-layer lists read}, the
+ * counts from which algorithmic HBM bytes follow. No reference counterpart. */
+int hnswgpu_read_stats(hnswgpu_index* h, uint32_t* out, uint64_t nq);
+
+/* ------------------------------------------------------------------------ */
+/* hnsw::Index surface                                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct hnsw_index hnsw_index;
+
+/* Replaces: Index(const Params&) (include/hnsw/index.hpp:14-21,39). */
+int hnsw_index_create(uint64_t dim, int32_t metric, uint64_t M, uint64_t ef_construction, uint64_t max_elements,
+                      uint64_t seed, hnsw_index** out);
+/* Replaces: Index::load(path, mmap) (include/hnsw/index.hpp:86). */
+int hnsw_index_load(const char* path, int mmap, hnsw_index** out);
+void hnsw_index_free(hnsw_index* h);
+/* Replaces: Index::save (include/hnsw/index.hpp:83). */
+int hnsw_index_save(hnsw_index* h, const char* path);
+/* Replaces: Index::add_batch / add (include/hnsw/index.hpp:47-50); CPU builder. */
+int hnsw_index_add_batch(hnsw_index* h, const float* vecs, const uint64_t* labels, uint64_t n,
+                         uint64_t num_threads);
+/* Replaces: Index::search_batch, on the GPU (include/hnsw/index.hpp:59-60). */
+int hnsw_index_search_batch(hnsw_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,
+                            hnswgpu_neighbor* out, uint64_t num_threads, const uint64_t* filter_words,
+                            uint64_t filter_size);
+/* Replaces: Index::search (include/hnsw/index.hpp:55-56): writes up to k rows
+ * and their number to *n_out. */
+int hnsw_index_search(hnsw_index* h, const float* q, uint64_t k, uint64_t ef, const uint64_t* filter_words,
+                      uint64_t filter_size, hnswgpu_neighbor* out, uint64_t* n_out);
+/* Replaces: Index::mark_deleted / is_deleted / contains (include/hnsw/index.hpp:64-66). */
+int hnsw_index_mark_deleted(hnsw_index* h, uint64_t label);
+int hnsw_index_is_deleted(hnsw_index* h, uint64_t label, int* out);
+int hnsw_index_contains(hnsw_index* h, uint64_t label, int* out);
+/* Replaces: Index::set_ef / ef (include/hnsw/index.hpp:73-74). */
+int hnsw_index_set_ef(hnsw_index* h, uint64_t ef);
+/* Replaces: Index::get_vector (include/hnsw/index.hpp:81). out has dim floats. */
+int hnsw_index_get_vector(hnsw_index* h, uint64_t label, float* out);
+/* info[0..9] = size, dim, metric, M, ef_construction, max_elements, ef, max_level, read_only,
+ * memory_usage (include/hnsw/index.hpp:68-78). */
+int hnsw_index_info(hnsw_index* h, int64_t* info);
+/* The device layer handle of an index (uploaded on first use; refreshed after add). */
+int hnsw_index_device(hnsw_index* h, hnswgpu_index** out);
+
+/* ------------------------------------------------------------------------ */
+/* BruteForceIndex (include/hnsw/index.hpp:101-118)                          */
+/* ------------------------------------------------------------------------ */
+typedef struct hnsw_brute_force hnsw_brute_force;
+int hnsw_brute_force_create(uint64_t dim, int32_t metric, hnsw_brute_force** out);
+void hnsw_brute_force_free(hnsw_brute_force* h);
+int hnsw_brute_force_add(hnsw_brute_force* h, const float* vecs, const uint64_t* labels, uint64_t n);
+int hnsw_brute_force_search_batch(hnsw_brute_force* h, const float* qs, uint64_t nq, uint64_t k,
+                                  hnswgpu_neighbor* out, uint64_t num_threads, const uint64_t* filter_words,
+                                  uint64_t filter_size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HNSWGPU_H_ */
