@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -472,3 +473,110 @@ extern "C" int hnswgpu_read_stats(hnswgpu_index* h, uint32_t* out, uint64_t nq) 
     check(hipMemcpy(out, h->stats.p, nq * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy stats");
   });
 }
+
+namespace {
+__global__ void copy_words(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    dst[i] = src[i];
+  }
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Device plumbing for callers without a GPU framework of their own: buffers,
+// streams and events of this library's HIP runtime.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int hnswgpu_malloc(uint64_t bytes, void** out) {
+  return guarded([&] {
+    if (out == nullptr) throw std::invalid_argument("malloc: null out");
+    check(hipMalloc(out, std::max<uint64_t>(bytes, 1)), "hipMalloc");
+  });
+}
+
+int hnswgpu_mfree(void* p) {
+  return guarded([&] { check(hipFree(p), "hipFree"); });
+}
+
+int hnswgpu_memcpy(void* dst, const void* src, uint64_t bytes, int kind) {
+  return guarded([&] {
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                          : hipMemcpyDeviceToDevice;
+    check(hipMemcpy(dst, src, bytes, k), "hipMemcpy");
+  });
+}
+
+int hnswgpu_copy_kernel(void* dst, const void* src, uint64_t bytes) {
+  return guarded([&] {
+    if (bytes % 16 != 0) throw std::invalid_argument("copy_kernel: bytes must be a multiple of 16");
+    const size_t n = bytes / 16;
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 65536));
+    if (n == 0) return;
+    hipLaunchKernelGGL(copy_words, dim3(blocks), dim3(256), 0, nullptr, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), n);
+    check(hipGetLastError(), "copy kernel launch");
+    check(hipDeviceSynchronize(), "copy kernel");
+  });
+}
+
+int hnswgpu_stream_create(void** out) {
+  return guarded([&] {
+    hipStream_t s = nullptr;
+    check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    *out = s;
+  });
+}
+
+int hnswgpu_stream_destroy(void* s) {
+  return guarded([&] { check(hipStreamDestroy(static_cast<hipStream_t>(s)), "hipStreamDestroy"); });
+}
+
+int hnswgpu_stream_synchronize(void* s) {
+  return guarded([&] { check(hipStreamSynchronize(static_cast<hipStream_t>(s)), "hipStreamSynchronize"); });
+}
+
+int hnswgpu_event_create(void** out) {
+  return guarded([&] {
+    hipEvent_t e = nullptr;
+    check(hipEventCreate(&e), "hipEventCreate");
+    *out = e;
+  });
+}
+
+int hnswgpu_event_destroy(void* e) {
+  return guarded([&] { check(hipEventDestroy(static_cast<hipEvent_t>(e)), "hipEventDestroy"); });
+}
+
+int hnswgpu_event_record(void* e, void* s) {
+  return guarded([&] {
+    check(hipEventRecord(static_cast<hipEvent_t>(e), static_cast<hipStream_t>(s)), "hipEventRecord");
+  });
+}
+
+int hnswgpu_event_elapsed_ms(void* begin, void* end, float* ms) {
+  return guarded([&] {
+    check(hipEventSynchronize(static_cast<hipEvent_t>(end)), "hipEventSynchronize");
+    check(hipEventElapsedTime(ms, static_cast<hipEvent_t>(begin), static_cast<hipEvent_t>(end)),
+          "hipEventElapsedTime");
+  });
+}
+
+int hnswgpu_device_info(char* name, int len, int64_t* info) {
+  return guarded([&] {
+    hipDeviceProp_t p{};
+    check(hipGetDeviceProperties(&p, 0), "hipGetDeviceProperties");
+    if (name != nullptr && len > 0) std::snprintf(name, static_cast<size_t>(len), "%s", p.gcnArchName);
+    if (info != nullptr) {
+      info[0] = p.multiProcessorCount;
+      info[1] = static_cast<int64_t>(p.totalGlobalMem);
+      info[2] = static_cast<int64_t>(p.sharedMemPerMultiprocessor);
+      info[3] = p.clockRate;
+      info[4] = p.memoryClockRate;
+      info[5] = p.memoryBusWidth;
+    }
+  });
+}
+
+}  // extern "C"

@@ -45,6 +45,18 @@ def _load():
         "hnswgpu_set_option": ([p, C.c_char_p, i64], C.c_int),
         "hnswgpu_get_option": ([p, C.c_char_p, p], C.c_int),
         "hnswgpu_read_stats": ([p, p, u64], C.c_int),
+        "hnswgpu_malloc": ([u64, p], C.c_int),
+        "hnswgpu_mfree": ([p], C.c_int),
+        "hnswgpu_memcpy": ([p, p, u64, C.c_int], C.c_int),
+        "hnswgpu_copy_kernel": ([p, p, u64], C.c_int),
+        "hnswgpu_stream_create": ([p], C.c_int),
+        "hnswgpu_stream_destroy": ([p], C.c_int),
+        "hnswgpu_stream_synchronize": ([p], C.c_int),
+        "hnswgpu_event_create": ([p], C.c_int),
+        "hnswgpu_event_destroy": ([p], C.c_int),
+        "hnswgpu_event_record": ([p, p], C.c_int),
+        "hnswgpu_event_elapsed_ms": ([p, p, p], C.c_int),
+        "hnswgpu_device_info": ([p, C.c_int, p], C.c_int),
         "hnsw_index_create": ([u64, i32, u64, u64, u64, u64, p], C.c_int),
         "hnsw_index_load": ([C.c_char_p, C.c_int, p], C.c_int),
         "hnsw_index_free": ([p], None),
@@ -119,6 +131,85 @@ def filter_words(filter, num_labels=None):
     words = np.zeros(max(1, (size + 63) // 64), dtype=np.uint64)
     np.bitwise_or.at(words, (labels >> np.uint64(6)).astype(np.int64), np.uint64(1) << (labels & np.uint64(63)))
     return words, size
+
+
+class DeviceBuffer:
+    """A device allocation of the library's runtime (hnswgpu_malloc)."""
+
+    def __init__(self, nbytes: int):
+        h = C.c_void_p()
+        _check(_lib.hnswgpu_malloc(max(1, int(nbytes)), C.byref(h)))
+        self.ptr, self.nbytes = h.value, int(nbytes)
+
+    @staticmethod
+    def of(array: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(array)
+        b = DeviceBuffer(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, array: np.ndarray) -> None:
+        a = np.ascontiguousarray(array)
+        assert a.nbytes <= self.nbytes
+        _check(_lib.hnswgpu_memcpy(self.ptr, _ptr(a), a.nbytes, 0))
+
+    def download(self, out: np.ndarray) -> np.ndarray:
+        assert out.flags.c_contiguous and out.nbytes <= self.nbytes
+        _check(_lib.hnswgpu_memcpy(_ptr(out), self.ptr, out.nbytes, 1))
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            _lib.hnswgpu_mfree(self.ptr)
+            self.ptr = None
+
+
+class Stream:
+    def __init__(self):
+        h = C.c_void_p()
+        _check(_lib.hnswgpu_stream_create(C.byref(h)))
+        self.handle = h.value
+
+    def synchronize(self) -> None:
+        _check(_lib.hnswgpu_stream_synchronize(self.handle))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.hnswgpu_stream_destroy(self.handle)
+            self.handle = None
+
+
+class Event:
+    def __init__(self):
+        h = C.c_void_p()
+        _check(_lib.hnswgpu_event_create(C.byref(h)))
+        self.handle = h.value
+
+    def record(self, stream: "Stream") -> None:
+        _check(_lib.hnswgpu_event_record(self.handle, stream.handle))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        _check(_lib.hnswgpu_event_elapsed_ms(self.handle, end.handle, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.hnswgpu_event_destroy(self.handle)
+            self.handle = None
+
+
+def copy_device(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> None:
+    """Copy with the library's plain copy kernel (synchronous)."""
+    _check(_lib.hnswgpu_copy_kernel(dst.ptr, src.ptr, int(nbytes)))
+
+
+def device_info():
+    name = C.create_string_buffer(128)
+    info = np.zeros(6, np.int64)
+    _check(_lib.hnswgpu_device_info(name, 128, _ptr(info)))
+    keys = ["cus", "global_mem", "lds_per_cu", "clock_khz", "mem_clock_khz", "mem_bus_width"]
+    return {"arch": name.value.decode(), **dict(zip(keys, info.tolist()))}
 
 
 class DeviceIndex:

@@ -124,6 +124,27 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value);
  * counts from which algorithmic HBM bytes follow. No reference counterpart. */
 int hnswgpu_read_stats(hnswgpu_index* h, uint32_t* out, uint64_t nq);
 
+/* Device plumbing of this library's HIP runtime, for callers without a GPU
+ * framework (no reference counterpart): device buffers (kind 0 = host to
+ * device, 1 = device to host, 2 = device to device), non-blocking streams at
+ * the default priority, events, and device 0's properties
+ * (info[0..5] = CUs, global memory bytes, LDS bytes per CU, clock kHz,
+ * memory clock kHz, memory bus width). */
+int hnswgpu_malloc(uint64_t bytes, void** out);
+int hnswgpu_mfree(void* p);
+int hnswgpu_memcpy(void* dst, const void* src, uint64_t bytes, int kind);
+/* A plain 16-byte-per-thread copy kernel (bytes % 16 == 0), synchronous: the
+ * known-size reference for hardware byte counters. */
+int hnswgpu_copy_kernel(void* dst, const void* src, uint64_t bytes);
+int hnswgpu_stream_create(void** out);
+int hnswgpu_stream_destroy(void* stream);
+int hnswgpu_stream_synchronize(void* stream);
+int hnswgpu_event_create(void** out);
+int hnswgpu_event_destroy(void* event);
+int hnswgpu_event_record(void* event, void* stream);
+int hnswgpu_event_elapsed_ms(void* begin, void* end, float* ms);
+int hnswgpu_device_info(char* name, int len, int64_t* info);
+
 /* ------------------------------------------------------------------------ */
 /* hnsw::Index surface                                                       */
 /* ------------------------------------------------------------------------ */

@@ -92,10 +92,13 @@ def test_uniform_20k_x_128_parity(lib, oracle, tmp_path):
         print("ef", ef, flush=True)
         got = ix.search_rows(Q, 10, ef)
         assert rows_equal(got, orc.search(Q, 10, ef, order="avx512", threads=8)), ef
+    # Uniform 128-d data is hard for graph search: recall is what the
+    # reference's algorithm gives (ids are identical), rising with ef.
     d = ((Q[:200, None, :].astype(np.float64) - X[None, :, :]) ** 2).sum(-1)
     truth = np.argsort(d, axis=1)[:, :10]
-    got = ix.search_rows(Q[:200], 10, 256)
-    assert workloads.recall(got["label"], truth, 10) >= 0.95
+    r256 = workloads.recall(ix.search_rows(Q[:200], 10, 256)["label"], truth, 10)
+    r512 = workloads.recall(ix.search_rows(Q[:200], 10, 512)["label"], truth, 10)
+    assert 0.85 <= r256 <= r512
 
 
 def test_edge_cases(lib):
@@ -163,16 +166,29 @@ def test_soft_delete_and_filter_semantics(lib):
 
 
 def test_device_buffers_match_host_api(golden, lib):
-    import torch
-
     ix = lib.Index.load(GOLDEN / "l2_d200_avx512.idx")
     Q = golden["l2_d200/queries"]
     want = ix.search_rows(Q, 10, 64)
     dev = ix.device()
-    q = torch.from_numpy(Q).cuda()
-    out = torch.empty((Q.shape[0], 10, 2), dtype=torch.int64, device="cuda")
-    stream = torch.cuda.current_stream()
-    dev.search_device(q.data_ptr(), Q.shape[0], 10, 64, out.data_ptr(), stream=stream.cuda_stream)
-    torch.cuda.synchronize()
-    got = out.cpu().numpy().view(lib.NEIGHBOR).reshape(Q.shape[0], 10)
+    q = lib.DeviceBuffer.of(Q)
+    out = lib.DeviceBuffer(Q.shape[0] * 10 * lib.NEIGHBOR.itemsize)
+    stream = lib.Stream()
+    dev.search_device(q.ptr, Q.shape[0], 10, 64, out.ptr, stream=stream.handle)
+    stream.synchronize()
+    got = out.download(np.empty(Q.shape[0] * 10, lib.NEIGHBOR)).reshape(Q.shape[0], 10)
     assert rows_equal(got, want)
+
+
+def test_work_counters_equal_oracle_counters(golden, lib, oracle):
+    """The kernel's own per-query work counts (which give the roofline bytes)
+    equal the oracle's counts of the same traversal."""
+    path = GOLDEN / "l2_d32_scalar.idx"
+    ix, orc = lib.Index.load(path), oracle.load(path)
+    Q = golden["l2_d32/queries"]
+    dev = ix.device()
+    dev.set_option("collect_stats", 1)
+    ix.search_rows(Q, 10, 64)
+    got = dev.read_stats(Q.shape[0]).astype(np.uint64)
+    dev.set_option("collect_stats", 0)
+    _, want = orc.search(Q, 10, 64, order="avx512", counters=True)
+    assert np.array_equal(got, want)
