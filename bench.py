@@ -208,16 +208,21 @@ def main() -> int:
         stats = dev.read_stats(args.nq).astype(np.float64)
         dev.set_option("collect_stats", 0)
         overflow = dev.get_option("last_overflow")
+        spilled = dev.get_option("last_hbm_visited")
+        handoff_c = dev.get_option("last_handoff_candidates")
         n_dist, n_exp0, n_expu = stats.mean(axis=0)
         bytes_q = n_dist * 128 * 4 + n_exp0 * 4 * (1 + 32) + n_expu * 4 * (1 + 16) + 128 * 4 + k * 16
         kernel_s = float(np.mean(kns)) / 1e9
         gbs = bytes_q * args.nq / kernel_s / 1e9
         per_ef[ef] = {"ef": ef, "recall": round(rec, 5), "qps": round(qps, 1), "ms_per_step": round(ms, 4),
                       "kernel_ms": round(kernel_s * 1e3, 4), "bytes_per_query": round(bytes_q, 1),
-                      "dist_per_query": round(n_dist, 1), "hbm_gbs": round(gbs, 1)}
+                      "dist_per_query": round(n_dist, 1), "hbm_gbs": round(gbs, 1),
+                      "general_kernel_queries": int(overflow), "hbm_visited_queries": int(spilled),
+                      "candidate_handoffs": int(handoff_c)}
         points.append((ef, rec, qps))
         log(f"gpu ef={ef} recall={rec:.4f} qps={qps:,.0f} step={ms:.3f}ms kernel={kernel_s * 1e3:.3f}ms "
-            f"bytes/q={bytes_q:,.0f} achieved={gbs:,.0f} GB/s dist/q={n_dist:.0f}")
+            f"bytes/q={bytes_q:,.0f} achieved={gbs:,.0f} GB/s dist/q={n_dist:.0f} general={overflow} "
+            f"hbm-visited={spilled}")
 
     if args.profile:
         print(json.dumps({"profile": True, "points": list(per_ef.values())}), flush=True)

@@ -80,6 +80,13 @@ struct hnswgpu_index {
   // scratch
   DevBuf counters, overflow, qprep, elig, spill_bits, spill_heaps, host_qs, host_out, host_counts, host_filter;
   uint32_t spill_waves = 64;
+  // HBM second level of the visited set: one table per resident wavefront,
+  // kept empty between queries by the kernel itself.
+  DevBuf visit_tables;
+  size_t visit_tables_cleared = 0;
+  uint32_t visit_log2 = 15;
+  uint32_t waves_per_cu = 16;
+  uint32_t last_counters[8] = {};
   uint64_t spill_alloc_count = 0;
 
   // options
@@ -96,7 +103,8 @@ struct hnswgpu_index {
 
   ~hnswgpu_index() {
     for (DevBuf* b : {&vecs, &l0, &upper, &upper_off, &labels, &deleted, &counters, &overflow, &qprep, &elig,
-                      &spill_bits, &spill_heaps, &host_qs, &host_out, &host_counts, &host_filter}) {
+                      &spill_bits, &spill_heaps, &host_qs, &host_out, &host_counts, &host_filter,
+                      &visit_tables}) {
       b->release();
     }
     stats.release();
@@ -194,10 +202,20 @@ void upload_view(hnswgpu_index& h, const hnswgpu_index_view& v) {
   h.upper_used = v.upper_used;
 }
 
-/// Visited-hash size: about 48 slots per unit of ef, within [2^11, 2^14].
+/// LDS visited-hash size: about 16 slots per unit of ef, within [2^11, 2^13];
+/// queries that visit more continue in the HBM table.
 uint32_t auto_hash_log2(uint32_t ef) {
-  const uint32_t l = next_log2(static_cast<uint64_t>(ef) * 48);
-  return std::min<uint32_t>(14, std::max<uint32_t>(11, l));
+  const uint32_t l = next_log2(static_cast<uint64_t>(ef) * 16);
+  return std::min<uint32_t>(13, std::max<uint32_t>(11, l));
+}
+
+/// Wavefronts resident at once for a kernel using `lds_per_wave` bytes of LDS
+/// and at most `waves_per_cu` waves per CU by registers (MI355X: 256 CUs,
+/// 160 KiB LDS each).
+uint32_t resident_waves(size_t lds_per_wave, uint32_t wpb, uint32_t waves_per_cu) {
+  const size_t per_block = lds_per_wave * wpb;
+  const uint32_t by_lds = static_cast<uint32_t>((160 * 1024) / per_block) * wpb;
+  return 256u * std::max<uint32_t>(wpb, std::min(by_lds, waves_per_cu));
 }
 
 void ensure_spill(hnswgpu_index& h) {
@@ -218,7 +236,7 @@ void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint
     throw std::invalid_argument("search_batch: size out of range");
   }
   const DevIndexView& ix = h.view;
-  auto* counters = static_cast<uint32_t*>(h.counters.need(4 * sizeof(uint32_t)));
+  auto* counters = static_cast<uint32_t*>(h.counters.need(8 * sizeof(uint32_t)));
   SearchScratch s{};
   s.counters = counters;
   s.overflow = static_cast<uint32_t*>(h.overflow.need(nq * sizeof(uint32_t)));
@@ -244,7 +262,20 @@ void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint
   r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(r.ef);
   const size_t per_wave = (static_cast<size_t>(1u << r.hash_log2) + 192u + ix.padded) * 4u;
   r.waves_per_block = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / per_wave)));
-  r.blocks = static_cast<uint32_t>(std::min<uint64_t>((nq + r.waves_per_block - 1) / r.waves_per_block, 8192));
+  // A persistent grid of the resident wavefronts (each owns an HBM visited table).
+  const uint32_t waves = std::min<uint32_t>(resident_waves(per_wave, r.waves_per_block, h.waves_per_cu),
+                                            static_cast<uint32_t>((nq + r.waves_per_block - 1) /
+                                                                  r.waves_per_block * r.waves_per_block));
+  r.blocks = std::max<uint32_t>(1, waves / r.waves_per_block);
+  const size_t table_words = static_cast<size_t>(1) << h.visit_log2;
+  s.visit_log2 = h.visit_log2;
+  s.visit_tables = static_cast<uint32_t*>(
+      h.visit_tables.need(static_cast<size_t>(r.blocks) * r.waves_per_block * table_words * sizeof(uint32_t)));
+  if (h.visit_tables_cleared < h.visit_tables.bytes) {
+    check(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(h.visit_tables.p), 0xFFFFFFFFu,
+                            h.visit_tables.bytes / sizeof(uint32_t), st), "clear visited tables");
+    h.visit_tables_cleared = h.visit_tables.bytes;
+  }
 
   const bool filtered = d_fw != nullptr || h.num_deleted > 0;
   r.elig = nullptr;
@@ -368,10 +399,11 @@ void device_search_host(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t
     if (counts != nullptr) {
       check(hipMemcpyAsync(counts, d_counts, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "hipMemcpy counts");
     }
-    uint32_t ctr[4] = {};
+    uint32_t ctr[8] = {};
     check(hipMemcpyAsync(ctr, h->counters.p, sizeof(ctr), hipMemcpyDeviceToHost, st), "hipMemcpy counters");
     check(hipStreamSynchronize(st), "search_batch");
     h->last_overflow = ctr[1];
+    std::copy(ctr, ctr + 8, h->last_counters);
 }
 
 }  // namespace hnswamd
@@ -411,6 +443,14 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
       h->hash_log2 = static_cast<uint32_t>(value);
     } else if (n == "force_spill") {
       h->force_spill = value != 0 ? 1 : 0;
+    } else if (n == "visit_log2") {
+      if (value < 10 || value > 24) throw std::invalid_argument("visit_log2 must be 10..24");
+      h->visit_log2 = static_cast<uint32_t>(value);
+      h->visit_tables.release();
+      h->visit_tables_cleared = 0;
+    } else if (n == "waves_per_cu") {
+      if (value < 1 || value > 32) throw std::invalid_argument("waves_per_cu must be 1..32");
+      h->waves_per_cu = static_cast<uint32_t>(value);
     } else if (n == "collect_stats") {
       h->collect_stats = value != 0 ? 1 : 0;
     } else if (n == "kernel_timing") {
@@ -438,8 +478,35 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
       *value = h->force_spill;
     } else if (n == "spill_waves") {
       *value = h->spill_waves;
+    } else if (n.rfind("last_", 0) == 0 && n != "last_kernel_ns" && h->counters.p != nullptr) {
+      // Counters of the most recent search on this handle (waits for it).
+      check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      check(hipMemcpy(h->last_counters, h->counters.p, sizeof(h->last_counters), hipMemcpyDeviceToHost),
+            "hipMemcpy counters");
+      h->last_overflow = h->last_counters[1];
+      if (n == "last_overflow") {
+        *value = h->last_counters[1];
+      } else if (n == "last_hbm_visited") {
+        *value = h->last_counters[3];
+      } else if (n == "last_handoff_candidates") {
+        *value = h->last_counters[4];
+      } else if (n == "last_handoff_visited") {
+        *value = h->last_counters[5];
+      } else {
+        throw std::invalid_argument("unknown option " + n);
+      }
     } else if (n == "last_overflow") {
       *value = h->last_overflow;
+    } else if (n == "last_hbm_visited") {
+      *value = h->last_counters[3];
+    } else if (n == "last_handoff_candidates") {
+      *value = h->last_counters[4];
+    } else if (n == "last_handoff_visited") {
+      *value = h->last_counters[5];
+    } else if (n == "visit_log2") {
+      *value = h->visit_log2;
+    } else if (n == "waves_per_cu") {
+      *value = h->waves_per_cu;
     } else if (n == "collect_stats") {
       *value = h->collect_stats;
     } else if (n == "kernel_timing") {

@@ -31,10 +31,14 @@ struct OutRow {
   uint32_t pad;
 };
 
-/// Scratch the kernels share: [0] query ticket, [1] overflow count,
-/// [2] spill ticket, [3] spare.
+/// Scratch the kernels share: [0] query ticket, [1] queries queued for the
+/// general kernel, [2] general-kernel ticket, [3] queries whose visited set
+/// went past the LDS hash into HBM, [4] queries queued because C dropped a
+/// live entry, [5] queries queued because the HBM visited table filled.
 struct SearchScratch {
-  uint32_t* counters;        // 4 words
+  uint32_t* counters;        // 8 words
+  uint32_t* visit_tables;    // one HBM visited table per resident wavefront
+  uint32_t visit_log2;       // slots per table = 1 << visit_log2
   uint32_t* overflow;        // capacity >= nq
   float* qprep;              // nq x padded
   uint32_t* spill_words;     // spill_waves x visited-bitmap words

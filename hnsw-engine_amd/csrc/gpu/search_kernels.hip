@@ -101,12 +101,71 @@ __device__ __forceinline__ bool hash_visit(uint32_t* tab, uint32_t log2h, uint32
   }
 }
 
-/// Greedy descent from `cur` through layers max_level..1.
+/// True iff id is in the LDS hash (no insertion).
+__device__ __forceinline__ bool hash_has(const uint32_t* tab, uint32_t log2h, uint32_t id) {
+  const uint32_t mask = (1u << log2h) - 1u;
+  uint32_t h = (id * 0x9E3779B1u) >> (32u - log2h);
+  for (;;) {
+    const uint32_t v = tab[h];
+    if (v == id) return true;
+    if (v == kEmptySlot) return false;
+    h = (h + 1u) & mask;
+  }
+}
+
+/// Exact visited set of one query: an open-addressing hash in LDS and, once
+/// that reaches 3/4 load, a second one in HBM (one table per resident
+/// wavefront, emptied after a query that used it). After the switch new ids go
+/// to the HBM table; lookups check LDS first.
+struct Visited {
+  uint32_t* lds;
+  uint32_t lds_log2;
+  uint32_t* hbm;  // null: no second level
+  uint32_t hbm_log2;
+  uint32_t in_lds;
+  uint32_t in_hbm;
+  bool spilled;
+
+  /// Room for `more` new ids? Switches to the HBM table when the LDS one is
+  /// full; false when both are.
+  __device__ __forceinline__ bool reserve(uint32_t more) {
+    if (!spilled) {
+      if (in_lds + more <= ((1u << lds_log2) >> 2) * 3) return true;
+      if (hbm == nullptr) return false;
+      spilled = true;
+    }
+    return in_hbm + more <= ((1u << hbm_log2) >> 2) * 3;
+  }
+  /// Per-lane test-and-set; every lane holding an id calls it together.
+  __device__ __forceinline__ bool visit(uint32_t id) {
+    if (!spilled) return hash_visit(lds, lds_log2, id);
+    if (hash_has(lds, lds_log2, id)) return false;
+    return hash_visit(hbm, hbm_log2, id);
+  }
+  __device__ __forceinline__ void count(uint32_t fresh) {
+    if (spilled) {
+      in_hbm += fresh;
+    } else {
+      in_lds += fresh;
+    }
+  }
+  /// Empty the HBM table if this query used it.
+  __device__ __forceinline__ void finish() {
+    if (!spilled) return;
+    uint4* t = reinterpret_cast<uint4*>(hbm);
+    const uint4 e = make_uint4(kEmptySlot, kEmptySlot, kEmptySlot, kEmptySlot);
+    for (uint32_t i = static_cast<uint32_t>(lane_id()); i < ((1u << hbm_log2) >> 2); i += 64) t[i] = e;
+    __threadfence();
+    spilled = false;
+  }
+};
+
 /// Per-query work counts (distances, layer-0 list reads, upper-layer list reads).
 struct Work {
   uint32_t dist = 0, exp0 = 0, expu = 0;
 };
 
+/// Greedy descent from `cur` through layers max_level..1.
 template <int PC>
 __device__ uint64_t descend(const DevIndexView& ix, const WaveLds& w, uint64_t cur, Work& work) {
   const int lane = lane_id();
@@ -149,17 +208,34 @@ __device__ uint64_t start_query(const DevIndexView& ix, const SearchScratch& s, 
   return descend<PC>(ix, w, cur, work);
 }
 
-/// Neighbours of `cid` on layer 0 that are new to the visited set, in list
-/// order: ids into w.nid, eligibility into w.nel. Returns their number; the
-/// visit itself is done by `visit(lane_has_id, id)`.
-template <typename Visit>
-__device__ __forceinline__ uint32_t fresh_neighbours(const DevIndexView& ix, const WaveLds& w, uint32_t cid,
-                                                     const uint32_t* elig, Visit visit) {
+/// A node's layer-0 list in registers: lane j holds entry j + 1, `cnt` the
+/// length. One load per lane, so it can be issued ahead of its use.
+struct ListRegs {
+  uint32_t id;
+  uint32_t cnt;
+  uint32_t nb;
+};
+
+__device__ __forceinline__ ListRegs load_list0(const DevIndexView& ix, uint32_t id) {
   const int lane = lane_id();
-  const uint32_t* l = ix.l0 + static_cast<size_t>(cid) * (1u + ix.M0);
-  const uint32_t cnt = uniform(l[0]);
+  const uint32_t* l = ix.l0 + static_cast<size_t>(id) * (1u + ix.M0);
+  ListRegs r;
+  r.id = id;
+  r.cnt = l[0];
+  r.nb = static_cast<uint32_t>(lane) < ix.M0 ? l[1 + lane] : 0u;
+  return r;
+}
+
+/// Neighbours in `lr` that are new to the visited set, in list order: ids into
+/// w.nid, eligibility into w.nel. Returns their number; the test-and-set is
+/// `visit(id)`, called by all lanes holding an id at once.
+template <typename Visit>
+__device__ __forceinline__ uint32_t fresh_from_list(const DevIndexView& ix, const WaveLds& w, const ListRegs& lr,
+                                                    const uint32_t* elig, Visit visit) {
+  const int lane = lane_id();
+  const uint32_t cnt = uniform(lr.cnt);
   bool has = static_cast<uint32_t>(lane) < cnt;
-  const uint32_t nb = has ? l[1 + lane] : 0u;
+  const uint32_t nb = lr.nb;
   if (ix.has_dups) {
     // A repeated id counts as visited at its later positions.
     for (uint32_t i = 0; i + 1 < cnt; ++i) {
@@ -179,23 +255,32 @@ __device__ __forceinline__ uint32_t fresh_neighbours(const DevIndexView& ix, con
   return static_cast<uint32_t>(__popcll(fm));
 }
 
+template <typename Visit>
+__device__ __forceinline__ uint32_t fresh_neighbours(const DevIndexView& ix, const WaveLds& w, uint32_t cid,
+                                                     const uint32_t* elig, Visit visit) {
+  return fresh_from_list(ix, w, load_list0(ix, cid), elig, visit);
+}
+
 __device__ __forceinline__ void write_pad(OutRow* row, uint32_t from, uint32_t k) {
   for (uint32_t j = from + static_cast<uint32_t>(lane_id()); j < k; j += 64) {
     row[j] = OutRow{~0ULL, __uint_as_float(0x7F800000u), 0u};
   }
 }
 
+/// Why the fast kernel handed a query to the general kernel.
+enum class Handoff : uint32_t { kNone = 0, kCandidates = 4, kVisited = 5 };
+
 template <int ER, int EC, int PC>
-__device__ bool fast_query(const DevIndexView& ix, const SearchRequest& r, const SearchScratch& s,
-                           const WaveLds& w, uint32_t qi) {
+__device__ Handoff fast_query(const DevIndexView& ix, const SearchRequest& r, const SearchScratch& s,
+                              const WaveLds& w, Visited& vis, uint32_t qi) {
   const int lane = lane_id();
-  const uint32_t log2h = r.hash_log2;
-  const uint32_t H = 1u << log2h;
   const uint32_t ccap = 64u * EC;
   const uint32_t ef = r.ef;
   const bool filtered = r.elig != nullptr;
 
-  for (uint32_t i = static_cast<uint32_t>(lane); i < H; i += 64) w.hash[i] = kEmptySlot;
+  for (uint32_t i = static_cast<uint32_t>(lane); i < (1u << vis.lds_log2); i += 64) w.hash[i] = kEmptySlot;
+  vis.in_lds = 0;
+  vis.in_hbm = 0;
   Work work;
   const uint64_t ep = start_query<PC>(ix, s, w, qi, work);
 
@@ -203,10 +288,11 @@ __device__ bool fast_query(const DevIndexView& ix, const SearchRequest& r, const
   WaveSorted<EC> C;
   R.clear();
   C.clear();
-  uint32_t sR = 0, sC = 0, tau = kOrdInf, nvis = 1;
+  uint32_t sR = 0, sC = 0, tau = kOrdInf;
   uint64_t drop_min = kKeyInf;
   wave_sync();
-  if (lane == 0) hash_visit(w.hash, log2h, key_id(ep));
+  if (lane == 0) hash_visit(w.hash, vis.lds_log2, key_id(ep));
+  vis.in_lds = 1;
   wave_sync();
   C.insert(ep, ccap);
   sC = 1;
@@ -215,22 +301,34 @@ __device__ bool fast_query(const DevIndexView& ix, const SearchRequest& r, const
     sR = 1;
     tau = key_ord(ep);
   }
+  // The list of the candidate expected next, loaded one expansion ahead.
+  ListRegs next = load_list0(ix, key_id(ep));
 
+  Handoff result = Handoff::kNone;
   for (;;) {
     const uint64_t c = C.at(0);
     if (drop_min < c) {
       if (sR >= ef && key_ord(drop_min) > tau) break;
-      return false;
+      result = Handoff::kCandidates;
+      break;
     }
     if (sC == 0) break;
     if (sR >= ef && key_ord(c) > tau) break;
     C.pop_front();
     --sC;
-    if (nvis + ix.M0 > (H >> 2) * 3) return false;
+    if (!vis.reserve(ix.M0)) {
+      result = Handoff::kVisited;
+      break;
+    }
+    const uint32_t cid = key_id(c);
+    const ListRegs cur = uniform(next.id) == cid ? next : load_list0(ix, cid);
+    // Prefetch the list of the candidate that is now smallest; it is usually
+    // the next one expanded.
+    const uint64_t peek = C.at(0);
+    if (peek != kKeyInf) next = load_list0(ix, key_id(peek));
 
-    const uint32_t U = fresh_neighbours(ix, w, key_id(c), r.elig,
-                                        [&](uint32_t id) { return hash_visit(w.hash, log2h, id); });
-    nvis += U;
+    const uint32_t U = fresh_from_list(ix, w, cur, r.elig, [&](uint32_t id) { return vis.visit(id); });
+    vis.count(U);
     work.dist += U;
     ++work.exp0;
     row_dists<PC>(ix, w, U);
@@ -253,6 +351,9 @@ __device__ bool fast_query(const DevIndexView& ix, const SearchRequest& r, const
       }
     }
   }
+  if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
+  vis.finish();
+  if (result != Handoff::kNone) return result;
 
   const uint32_t kout = sR < r.k ? sR : r.k;
   OutRow* row = r.out + static_cast<size_t>(qi) * r.k;
@@ -271,7 +372,7 @@ __device__ bool fast_query(const DevIndexView& ix, const SearchRequest& r, const
     r.stats[3 * static_cast<size_t>(qi) + 1] = work.exp0;
     r.stats[3 * static_cast<size_t>(qi) + 2] = work.expu;
   }
-  return true;
+  return Handoff::kNone;
 }
 
 template <int ER, int EC, int PC>
@@ -283,16 +384,21 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView 
   const uint32_t per_wave = H + 192u + ix.padded;
   uint32_t* base = lds + static_cast<size_t>(threadIdx.x >> 6) * per_wave;
   const WaveLds w{base, base + H, base + H + 64, base + H + 128, reinterpret_cast<float*>(base + H + 192)};
+  const size_t wave_slot = static_cast<size_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  Visited vis{w.hash, r.hash_log2,
+              s.visit_tables != nullptr ? s.visit_tables + (wave_slot << s.visit_log2) : nullptr,
+              s.visit_log2, 0u, 0u, false};
   for (;;) {
     uint32_t qi = 0;
     if (lane == 0) qi = atomicAdd(&s.counters[0], 1u);
     qi = uniform(qi);
     if (qi >= r.nq) break;
-    bool ok = false;
-    if (!r.force_spill) ok = fast_query<ER, EC, PC>(ix, r, s, w, qi);
-    if (!ok && lane == 0) {
+    Handoff h = Handoff::kCandidates;
+    if (!r.force_spill) h = fast_query<ER, EC, PC>(ix, r, s, w, vis, qi);
+    if (h != Handoff::kNone && lane == 0) {
       const uint32_t slot = atomicAdd(&s.counters[1], 1u);
       s.overflow[slot] = qi;
+      if (!r.force_spill) atomicAdd(&s.counters[static_cast<uint32_t>(h)], 1u);
     }
   }
 }
@@ -432,7 +538,7 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) general_search(DevIndexVi
 // Small kernels.
 // ---------------------------------------------------------------------------
 __global__ void reset_counters(uint32_t* c) {
-  if (threadIdx.x < 4) c[threadIdx.x] = 0u;
+  if (threadIdx.x < 8) c[threadIdx.x] = 0u;
 }
 
 /// One wavefront per query: copy into a zero-padded row, normalising for

@@ -67,16 +67,27 @@ def test_general_kernel_equals_fast_kernel(golden, lib):
         assert rows_equal(fast, slow), name
 
 
-def test_tiny_visited_hash_overflows_and_stays_exact(golden, lib, oracle):
+def test_visited_set_spills_to_hbm_and_stays_exact(golden, lib, oracle):
     path = GOLDEN / "l2_d32_scalar.idx"
     ix, orc = lib.Index.load(path), oracle.load(path)
     Q = golden["l2_d32/queries"]
+    want = orc.search(Q, 10, 64, order="avx512")
     dev = ix.device()
+    # A 256-slot LDS hash: queries continue in their HBM tables.
     dev.set_option("hash_log2", 8)
     rows, _ = dev.search_rows(Q, 10, 64)
-    assert dev.get_option("last_overflow") > 0
+    assert dev.get_option("last_hbm_visited") > 0
+    assert dev.get_option("last_overflow") == 0
+    assert rows_equal(rows, want)
+    # A 1024-slot HBM table as well: the rest go to the general kernel.
+    dev.set_option("visit_log2", 10)
+    rows, _ = dev.search_rows(Q, 10, 64)
+    assert dev.get_option("last_handoff_visited") > 0
+    assert rows_equal(rows, want)
+    dev.set_option("visit_log2", 15)
     dev.set_option("hash_log2", 0)
-    assert rows_equal(rows, orc.search(Q, 10, 64, order="avx512"))
+    rows, _ = dev.search_rows(Q, 10, 64)
+    assert rows_equal(rows, want)
 
 
 def test_uniform_20k_x_128_parity(lib, oracle, tmp_path):
