@@ -125,6 +125,8 @@ def main() -> int:
     ap.add_argument("--index", default="", help="reuse/save the built index at this path")
     ap.add_argument("--profile", action="store_true",
                     help="profiling run: no ground truth, latency or CPU baseline; one line with the timings")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="device-layer option for every search (e.g. hash_log2=12)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (counter calibration) and exit")
     args = ap.parse_args()
@@ -173,6 +175,9 @@ def main() -> int:
     log(f"ground truth in {time.perf_counter() - t0:.1f}s (FP64 check agreement {truth_check})")
 
     dev = ix.device()
+    for kv in args.set:
+        name, value = kv.split("=")
+        dev.set_option(name, int(value))
     stream = hnswgpu.Stream()
     d_q = hnswgpu.DeviceBuffer.of(Q)
     d_out = hnswgpu.DeviceBuffer(args.nq * k * hnswgpu.NEIGHBOR.itemsize)

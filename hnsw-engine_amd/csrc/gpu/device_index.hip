@@ -444,7 +444,7 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
     } else if (n == "force_spill") {
       h->force_spill = value != 0 ? 1 : 0;
     } else if (n == "visit_log2") {
-      if (value < 10 || value > 24) throw std::invalid_argument("visit_log2 must be 10..24");
+      if (value < 8 || value > 24) throw std::invalid_argument("visit_log2 must be 8..24");
       h->visit_log2 = static_cast<uint32_t>(value);
       h->visit_tables.release();
       h->visit_tables_cleared = 0;

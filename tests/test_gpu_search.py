@@ -79,8 +79,8 @@ def test_visited_set_spills_to_hbm_and_stays_exact(golden, lib, oracle):
     assert dev.get_option("last_hbm_visited") > 0
     assert dev.get_option("last_overflow") == 0
     assert rows_equal(rows, want)
-    # A 1024-slot HBM table as well: the rest go to the general kernel.
-    dev.set_option("visit_log2", 10)
+    # A 256-slot HBM table as well: the rest go to the general kernel.
+    dev.set_option("visit_log2", 8)
     rows, _ = dev.search_rows(Q, 10, 64)
     assert dev.get_option("last_handoff_visited") > 0
     assert rows_equal(rows, want)
