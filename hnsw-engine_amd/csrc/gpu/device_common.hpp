@@ -116,6 +116,52 @@ struct WaveSorted {
     }
     return out;
   }
+  /// insert() that also moves a per-slot flag word with its entry; x gets
+  /// `xflag`, and the flag of the element pushed out is stored in out_flag.
+  __device__ __forceinline__ uint64_t insert_flagged(uint64_t x, uint32_t cap, uint32_t (&fl)[E], uint32_t xflag,
+                                                     uint32_t& out_flag) {
+    const uint32_t pos = rank(x);
+    out_flag = 0;
+    if (pos >= cap) return x;
+    const uint64_t out = at(cap - 1);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (static_cast<uint32_t>(e) == ((cap - 1) >> 6)) out_flag = rdlane(fl[e], static_cast<int>((cap - 1) & 63));
+    }
+    const int lane = lane_id();
+#pragma unroll
+    for (int e = E - 1; e >= 0; --e) {
+      const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
+      const int src = lane == 0 ? 0 : lane - 1;
+      uint64_t up = from_lane64(v[e], src);
+      uint32_t fup = from_lane(fl[e], src);
+      if (e > 0) {
+        const uint64_t carry = rdlane64(v[e - 1], 63);
+        const uint32_t fcarry = rdlane(fl[e - 1], 63);
+        if (lane == 0) {
+          up = carry;
+          fup = fcarry;
+        }
+      }
+      uint64_t nv = v[e];
+      uint32_t nf = fl[e];
+      if (k == pos) {
+        nv = x;
+        nf = xflag;
+      }
+      if (k > pos) {
+        nv = up;
+        nf = fup;
+      }
+      if (k >= cap) {
+        nv = kKeyInf;
+        nf = 0;
+      }
+      v[e] = nv;
+      fl[e] = nf;
+    }
+    return out;
+  }
   /// Remove element 0.
   __device__ __forceinline__ void pop_front() {
     const int lane = lane_id();

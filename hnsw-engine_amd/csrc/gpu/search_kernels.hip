@@ -270,13 +270,164 @@ __device__ __forceinline__ void write_pad(OutRow* row, uint32_t from, uint32_t k
 /// Why the fast kernel handed a query to the general kernel.
 enum class Handoff : uint32_t { kNone = 0, kCandidates = 4, kVisited = 5 };
 
+/// Results of a finished query: R's first min(k, |R|) entries with labels,
+/// padding, result count and work counts.
+template <int ER>
+__device__ void write_results(const DevIndexView& ix, const SearchRequest& r, const WaveSorted<ER>& R, uint32_t sR,
+                              const Work& work, uint32_t qi) {
+  const int lane = lane_id();
+  const uint32_t kout = sR < r.k ? sR : r.k;
+  OutRow* row = r.out + static_cast<size_t>(qi) * r.k;
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    const uint32_t idx = static_cast<uint32_t>(e * 64 + lane);
+    if (idx < kout) {
+      const uint64_t key = R.v[e];
+      row[idx] = OutRow{ix.labels[key_id(key)], float_of(key_ord(key)), 0u};
+    }
+  }
+  write_pad(row, kout, r.k);
+  if (r.counts != nullptr && lane == 0) r.counts[qi] = kout;
+  if (r.stats != nullptr && lane == 0) {
+    r.stats[3 * static_cast<size_t>(qi)] = work.dist;
+    r.stats[3 * static_cast<size_t>(qi) + 1] = work.exp0;
+    r.stats[3 * static_cast<size_t>(qi) + 2] = work.expu;
+  }
+}
+
+/// Index of the first result slot not yet expanded, or ~0 if none.
+template <int ER>
+__device__ __forceinline__ uint32_t first_unexpanded(const uint32_t (&fl)[ER], uint32_t sR) {
+  uint32_t slot = ~0u;
+#pragma unroll
+  for (int e = ER - 1; e >= 0; --e) {
+    const uint64_t m = ballot(static_cast<uint32_t>(e * 64 + lane_id()) < sR && fl[e] == 0u);
+    if (m != 0) slot = static_cast<uint32_t>(e * 64 + __builtin_ctzll(m));
+  }
+  return slot;
+}
+
+/// Layer-0 search without filter or deletions. Here every admitted node joins
+/// both C and R, so C needs no array of its own: its live entries are the
+/// entries of R not yet expanded (a flag per slot) plus the entries pushed out
+/// of R whose distance ties the new worst result (`ties`, lane i holding the
+/// i-th smallest id; all have distance tau). Every entry of R orders before
+/// every tie, and an entry pushed out with a larger distance can never be
+/// expanded again, so popping "first unexpanded slot of R, else the smallest
+/// tie" is the exact Alg. 2 order, and the loop ends exactly when Alg. 2 would
+/// stop (the smallest remaining candidate is further than the worst result).
+template <int ER, int PC>
+__device__ Handoff plain_query(const DevIndexView& ix, const SearchRequest& r, const SearchScratch& s,
+                               const WaveLds& w, Visited& vis, uint32_t qi) {
+  const int lane = lane_id();
+  const uint32_t ef = r.ef;
+
+  for (uint32_t i = static_cast<uint32_t>(lane); i < (1u << vis.lds_log2); i += 64) w.hash[i] = kEmptySlot;
+  vis.in_lds = 0;
+  vis.in_hbm = 0;
+  Work work;
+  const uint64_t ep = start_query<PC>(ix, s, w, qi, work);
+
+  WaveSorted<ER> R;
+  R.clear();
+  uint32_t fl[ER];
+#pragma unroll
+  for (int e = 0; e < ER; ++e) fl[e] = 0u;
+  uint32_t ties = 0u, n_ties = 0u;
+  wave_sync();
+  if (lane == 0) hash_visit(w.hash, vis.lds_log2, key_id(ep));
+  vis.in_lds = 1;
+  wave_sync();
+  uint32_t dummy;
+  R.insert_flagged(ep, ef, fl, 0u, dummy);
+  uint32_t sR = 1, tau = key_ord(ep);
+  ListRegs next = load_list0(ix, key_id(ep));
+
+  Handoff result = Handoff::kNone;
+  for (;;) {
+    uint64_t c;
+    const uint32_t slot = first_unexpanded<ER>(fl, sR);
+    if (slot != ~0u) {
+      c = R.at(slot);
+#pragma unroll
+      for (int e = 0; e < ER; ++e) {
+        if (static_cast<uint32_t>(e) == (slot >> 6) && static_cast<uint32_t>(lane) == (slot & 63)) fl[e] = 1u;
+      }
+    } else if (n_ties > 0) {
+      c = key_of(tau, rdlane(ties, 0));
+      ties = from_lane(ties, lane == 63 ? 63 : lane + 1);
+      --n_ties;
+    } else {
+      break;
+    }
+    if (!vis.reserve(ix.M0)) {
+      result = Handoff::kVisited;
+      break;
+    }
+    const uint32_t cid = key_id(c);
+    const ListRegs cur = uniform(next.id) == cid ? next : load_list0(ix, cid);
+    const uint32_t peek = first_unexpanded<ER>(fl, sR);
+    if (peek != ~0u) next = load_list0(ix, key_id(R.at(peek)));
+
+    const uint32_t U = fresh_from_list(ix, w, cur, nullptr, [&](uint32_t id) { return vis.visit(id); });
+    vis.count(U);
+    work.dist += U;
+    ++work.exp0;
+    row_dists<PC>(ix, w, U);
+    wave_sync();
+    const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
+    const uint32_t id_u = w.nid[lane];
+    uint64_t cand = ballot(static_cast<uint32_t>(lane) < U && (sR < ef || od_u < tau));
+    while (cand != 0) {
+      const int i = __builtin_ctzll(cand);
+      cand &= cand - 1;
+      const uint32_t od = rdlane(od_u, i);
+      if (sR >= ef && od >= tau) continue;
+      const uint64_t key = key_of(od, rdlane(id_u, i));
+      uint32_t out_flag;
+      const uint64_t out = R.insert_flagged(key, ef, fl, 0u, out_flag);
+      if (sR < ef) {
+        ++sR;
+        tau = key_ord(R.at(sR - 1));
+        continue;
+      }
+      tau = key_ord(R.at(ef - 1));
+      if (key_ord(out) > tau) {
+        n_ties = 0;  // the old ties had the old, larger worst distance
+      } else if (out_flag == 0u) {
+        if (n_ties == 64) {
+          result = Handoff::kCandidates;
+          break;
+        }
+        const uint32_t tid = key_id(out);
+        const uint32_t pos =
+            static_cast<uint32_t>(__popcll(ballot(static_cast<uint32_t>(lane) < n_ties && ties < tid)));
+        const uint32_t up = from_lane(ties, lane == 0 ? 0 : lane - 1);
+        if (static_cast<uint32_t>(lane) > pos) ties = up;
+        if (static_cast<uint32_t>(lane) == pos) ties = tid;
+        ++n_ties;
+      }
+    }
+    if (result != Handoff::kNone) break;
+  }
+  if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
+  vis.finish();
+  if (result != Handoff::kNone) return result;
+  write_results<ER>(ix, r, R, sR, work, qi);
+  return Handoff::kNone;
+}
+
+/// Layer-0 search with an eligibility bitmap (filter and/or deletions): C and R
+/// are separate sorted arrays; C keeps its 64 * EC smallest entries and the
+/// smallest entry it ever dropped is remembered -- the search stays exact as
+/// long as that entry is never the smallest live candidate, otherwise the
+/// query goes to the general kernel.
 template <int ER, int EC, int PC>
-__device__ Handoff fast_query(const DevIndexView& ix, const SearchRequest& r, const SearchScratch& s,
-                              const WaveLds& w, Visited& vis, uint32_t qi) {
+__device__ Handoff filtered_query(const DevIndexView& ix, const SearchRequest& r, const SearchScratch& s,
+                                  const WaveLds& w, Visited& vis, uint32_t qi) {
   const int lane = lane_id();
   const uint32_t ccap = 64u * EC;
   const uint32_t ef = r.ef;
-  const bool filtered = r.elig != nullptr;
 
   for (uint32_t i = static_cast<uint32_t>(lane); i < (1u << vis.lds_log2); i += 64) w.hash[i] = kEmptySlot;
   vis.in_lds = 0;
@@ -296,12 +447,11 @@ __device__ Handoff fast_query(const DevIndexView& ix, const SearchRequest& r, co
   wave_sync();
   C.insert(ep, ccap);
   sC = 1;
-  if (!filtered || elig_bit(r.elig, key_id(ep))) {
+  if (elig_bit(r.elig, key_id(ep))) {
     R.insert(ep, ef);
     sR = 1;
     tau = key_ord(ep);
   }
-  // The list of the candidate expected next, loaded one expansion ahead.
   ListRegs next = load_list0(ix, key_id(ep));
 
   Handoff result = Handoff::kNone;
@@ -322,8 +472,6 @@ __device__ Handoff fast_query(const DevIndexView& ix, const SearchRequest& r, co
     }
     const uint32_t cid = key_id(c);
     const ListRegs cur = uniform(next.id) == cid ? next : load_list0(ix, cid);
-    // Prefetch the list of the candidate that is now smallest; it is usually
-    // the next one expanded.
     const uint64_t peek = C.at(0);
     if (peek != kKeyInf) next = load_list0(ix, key_id(peek));
 
@@ -333,49 +481,37 @@ __device__ Handoff fast_query(const DevIndexView& ix, const SearchRequest& r, co
     ++work.exp0;
     row_dists<PC>(ix, w, U);
     wave_sync();
-    for (uint32_t u = 0; u < U; ++u) {
-      const uint32_t od = uniform(w.nd[u]);
-      if (sR < ef || od < tau) {
-        const uint64_t key = key_of(od, uniform(w.nid[u]));
-        const uint64_t dropped = C.insert(key, ccap);
-        if (dropped == kKeyInf) {
-          ++sC;
-        } else if (dropped < drop_min) {
-          drop_min = dropped;
-        }
-        if (!filtered || uniform(w.nel[u]) != 0u) {
-          R.insert(key, ef);
-          if (sR < ef) ++sR;
-          tau = key_ord(R.at(sR - 1));
-        }
+    const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
+    const uint32_t id_u = w.nid[lane];
+    const uint32_t el_u = w.nel[lane];
+    uint64_t cand = ballot(static_cast<uint32_t>(lane) < U && (sR < ef || od_u < tau));
+    while (cand != 0) {
+      const int i = __builtin_ctzll(cand);
+      cand &= cand - 1;
+      const uint32_t od = rdlane(od_u, i);
+      if (sR >= ef && od >= tau) continue;
+      const uint64_t key = key_of(od, rdlane(id_u, i));
+      const uint64_t dropped = C.insert(key, ccap);
+      if (dropped == kKeyInf) {
+        ++sC;
+      } else if (dropped < drop_min) {
+        drop_min = dropped;
+      }
+      if (rdlane(el_u, i) != 0u) {
+        R.insert(key, ef);
+        if (sR < ef) ++sR;
+        tau = key_ord(R.at(sR - 1));
       }
     }
   }
   if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
   vis.finish();
   if (result != Handoff::kNone) return result;
-
-  const uint32_t kout = sR < r.k ? sR : r.k;
-  OutRow* row = r.out + static_cast<size_t>(qi) * r.k;
-#pragma unroll
-  for (int e = 0; e < ER; ++e) {
-    const uint32_t idx = static_cast<uint32_t>(e * 64 + lane);
-    if (idx < kout) {
-      const uint64_t key = R.v[e];
-      row[idx] = OutRow{ix.labels[key_id(key)], float_of(key_ord(key)), 0u};
-    }
-  }
-  write_pad(row, kout, r.k);
-  if (r.counts != nullptr && lane == 0) r.counts[qi] = kout;
-  if (r.stats != nullptr && lane == 0) {
-    r.stats[3 * static_cast<size_t>(qi)] = work.dist;
-    r.stats[3 * static_cast<size_t>(qi) + 1] = work.exp0;
-    r.stats[3 * static_cast<size_t>(qi) + 2] = work.expu;
-  }
+  write_results<ER>(ix, r, R, sR, work, qi);
   return Handoff::kNone;
 }
 
-template <int ER, int EC, int PC>
+template <int ER, int EC, int PC, bool kFiltered>
 __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView ix, SearchRequest r,
                                                                    SearchScratch s) {
   extern __shared__ uint32_t lds[];
@@ -394,7 +530,13 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView 
     qi = uniform(qi);
     if (qi >= r.nq) break;
     Handoff h = Handoff::kCandidates;
-    if (!r.force_spill) h = fast_query<ER, EC, PC>(ix, r, s, w, vis, qi);
+    if (!r.force_spill) {
+      if constexpr (kFiltered) {
+        h = filtered_query<ER, EC, PC>(ix, r, s, w, vis, qi);
+      } else {
+        h = plain_query<ER, PC>(ix, r, s, w, vis, qi);
+      }
+    }
     if (h != Handoff::kNone && lane == 0) {
       const uint32_t slot = atomicAdd(&s.counters[1], 1u);
       s.overflow[slot] = qi;
@@ -588,26 +730,33 @@ __global__ void build_eligibility(uint32_t count, const uint64_t* labels, const 
   elig[word] = out;
 }
 
-template <int ER, int EC, int PC>
+template <int ER, int EC, int PC, bool kFiltered>
 hipError_t run_fast(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
   const size_t lds = static_cast<size_t>(r.waves_per_block) * ((1u << r.hash_log2) + 192u + ix.padded) * 4u;
   static bool configured = false;
   if (!configured) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fast_search<ER, EC, PC>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fast_search<ER, EC, PC, kFiltered>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
     configured = true;
   }
-  hipLaunchKernelGGL((fast_search<ER, EC, PC>), dim3(r.blocks), dim3(r.waves_per_block * 64), lds, st, ix, r, s);
+  hipLaunchKernelGGL((fast_search<ER, EC, PC, kFiltered>), dim3(r.blocks), dim3(r.waves_per_block * 64), lds, st,
+                     ix, r, s);
   return hipGetLastError();
 }
 
-template <int PC>
+template <int PC, bool kFiltered>
 hipError_t pick_capacity(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
-  if (r.ef <= 64) return run_fast<1, 2, PC>(ix, r, s, st);
-  if (r.ef <= 128) return run_fast<2, 4, PC>(ix, r, s, st);
-  if (r.ef <= 256) return run_fast<4, 8, PC>(ix, r, s, st);
-  return run_fast<8, 16, PC>(ix, r, s, st);
+  if (r.ef <= 64) return run_fast<1, 2, PC, kFiltered>(ix, r, s, st);
+  if (r.ef <= 128) return run_fast<2, 4, PC, kFiltered>(ix, r, s, st);
+  if (r.ef <= 256) return run_fast<4, 8, PC, kFiltered>(ix, r, s, st);
+  return run_fast<8, 16, PC, kFiltered>(ix, r, s, st);
+}
+
+template <int PC>
+hipError_t pick_variant(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
+  if (r.elig != nullptr) return pick_capacity<PC, true>(ix, r, s, st);
+  return pick_capacity<PC, false>(ix, r, s, st);
 }
 
 }  // namespace
@@ -640,9 +789,9 @@ hipError_t launch_search(const DevIndexView& ix, const SearchRequest& r, SearchS
     if (e != hipSuccess) return e;
   }
   if (ix.padded == 128) {
-    e = pick_capacity<8>(ix, rr, s, st);
+    e = pick_variant<8>(ix, rr, s, st);
   } else {
-    e = pick_capacity<0>(ix, rr, s, st);
+    e = pick_variant<0>(ix, rr, s, st);
   }
   if (e != hipSuccess) return e;
   if (fast_begin != nullptr && fast_end != nullptr) {
