@@ -122,6 +122,7 @@ def main() -> int:
     ap.add_argument("--build-threads", type=int, default=0)
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp32-only", action="store_true", help="skip the bf16 / fp8 traversal sweeps")
     ap.add_argument("--index", default="", help="reuse/save the built index at this path")
     ap.add_argument("--profile", action="store_true",
                     help="profiling run: no ground truth, latency or CPU baseline; one line with the timings")
@@ -233,6 +234,35 @@ def main() -> int:
         print(json.dumps({"profile": True, "points": list(per_ef.values())}), flush=True)
         return 0
     value = qps_at(points)
+
+    # Reduced-precision traversal (bf16; fp8 e4m3 + scale) with exact fp32 re-rank.
+    reduced = {}
+    for storage, name in ((1, "bf16_rerank"), (2, "fp8_rerank")):
+        if args.fp32_only:
+            break
+        dev.set_option("storage", storage)
+        rp = []
+        for ef in efs:
+            for _ in range(args.warmup):
+                step(ef)
+            stream.synchronize()
+            start, end = hnswgpu.Event(), hnswgpu.Event()
+            start.record(stream)
+            for _ in range(args.steps):
+                step(ef)
+            end.record(stream)
+            stream.synchronize()
+            ms = start.elapsed_ms(end) / args.steps
+            rows = d_out.download(host_rows).reshape(args.nq, k)
+            rec = recall_at(rows["label"].astype(np.int64), truth, k)
+            rp.append((ef, rec, args.nq / (ms / 1e3)))
+            log(f"gpu {name} ef={ef} recall={rec:.4f} qps={args.nq / (ms / 1e3):,.0f} step={ms:.3f}ms")
+        dev.set_option("storage", 0)
+        v = qps_at(rp)
+        reduced[name] = {"value": round(v, 1) if math.isfinite(v) else None,
+                         "over_fp32": round(v / value, 3) if math.isfinite(v) and math.isfinite(value) else None,
+                         "points": [{"ef": e, "recall": round(r, 5), "qps": round(q, 1)} for e, r, q in rp]}
+
     # Roofline at the first ef that reaches the recall target.
     op = next((per_ef[e] for e, r, _ in points if r >= 0.95), per_ef[efs[-1]])
 
@@ -272,6 +302,7 @@ def main() -> int:
                      "kernel": "fast_search", "at_ef": op["ef"], "kernel_ms": op["kernel_ms"],
                      "bytes_per_query": op["bytes_per_query"]},
         "cpu_baseline": cpu,
+        **reduced,
         "latency_ms": {"single_query_p50": round(float(np.percentile(lat, 50)), 4),
                        "single_query_p99": round(float(np.percentile(lat, 99)), 4),
                        "at_ef": op["ef"]},

@@ -77,6 +77,7 @@ struct hnswgpu_index {
   std::vector<uint8_t> deleted_host;  // mirror of the device bits, for counting
 
   DevBuf vecs, l0, upper, upper_off, labels, deleted;
+  DevBuf vlow, vscale, vnorm;  // reduced-precision traversal copy (storage option)
   // scratch
   DevBuf counters, overflow, qprep, elig, spill_bits, spill_heaps, host_qs, host_out, host_counts, host_filter;
   uint32_t spill_waves = 64;
@@ -104,7 +105,7 @@ struct hnswgpu_index {
   ~hnswgpu_index() {
     for (DevBuf* b : {&vecs, &l0, &upper, &upper_off, &labels, &deleted, &counters, &overflow, &qprep, &elig,
                       &spill_bits, &spill_heaps, &host_qs, &host_out, &host_counts, &host_filter,
-                      &visit_tables}) {
+                      &visit_tables, &vlow, &vscale, &vnorm}) {
       b->release();
     }
     stats.release();
@@ -260,7 +261,7 @@ void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint
   h.last_nq = nq;
   r.force_spill = h.force_spill;
   r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(r.ef);
-  const size_t per_wave = (static_cast<size_t>(1u << r.hash_log2) + 192u + ix.padded) * 4u;
+  const size_t per_wave = static_cast<size_t>(hnswamd::gpu::lds_words_per_wave(r.hash_log2, ix.padded)) * 4u;
   r.waves_per_block = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / per_wave)));
   // A persistent grid of the resident wavefronts (each owns an HBM visited table).
   const uint32_t waves = std::min<uint32_t>(resident_waves(per_wave, r.waves_per_block, h.waves_per_cu),
@@ -295,6 +296,32 @@ void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint
   }
   check(hnswamd::gpu::launch_search(ix, r, s, st, b, e), "search kernel");
   h.timing_pending = b != nullptr;
+}
+
+/// Build (or drop) the reduced-precision traversal copy of the rows.
+void set_storage(hnswgpu_index& h, int storage) {
+  if (storage == h.view.storage) return;
+  h.view.storage = 0;
+  h.view.vlow = nullptr;
+  h.view.vscale = nullptr;
+  h.view.vnorm = nullptr;
+  h.vlow.release();
+  h.vscale.release();
+  h.vnorm.release();
+  if (storage == 0 || h.view.count == 0) {
+    h.view.storage = storage;
+    return;
+  }
+  const size_t n = h.view.count, p = h.view.padded;
+  void* low = h.vlow.need(n * p * (storage == 1 ? 2 : 1));
+  auto* scale = static_cast<float*>(h.vscale.need(n * sizeof(float)));
+  auto* norm = static_cast<float*>(h.vnorm.need(n * sizeof(float)));
+  check(hnswamd::gpu::launch_build_storage(h.view, storage, low, scale, norm, h.stream), "build storage");
+  check(hipStreamSynchronize(h.stream), "build storage");
+  h.view.storage = storage;
+  h.view.vlow = low;
+  h.view.vscale = scale;
+  h.view.vnorm = norm;
 }
 
 template <typename F>
@@ -443,6 +470,11 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
       h->hash_log2 = static_cast<uint32_t>(value);
     } else if (n == "force_spill") {
       h->force_spill = value != 0 ? 1 : 0;
+    } else if (n == "storage") {
+      // 0 = traverse the fp32 rows; 1 = a bf16 copy; 2 = an fp8 e4m3 copy with
+      // per-vector scales. Reduced copies re-rank the final candidates in fp32.
+      if (value < 0 || value > 2) throw std::invalid_argument("storage must be 0 (fp32), 1 (bf16) or 2 (fp8)");
+      set_storage(*h, static_cast<int>(value));
     } else if (n == "visit_log2") {
       if (value < 8 || value > 24) throw std::invalid_argument("visit_log2 must be 8..24");
       h->visit_log2 = static_cast<uint32_t>(value);
@@ -507,6 +539,8 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
       *value = h->visit_log2;
     } else if (n == "waves_per_cu") {
       *value = h->waves_per_cu;
+    } else if (n == "storage") {
+      *value = h->view.storage;
     } else if (n == "collect_stats") {
       *value = h->collect_stats;
     } else if (n == "kernel_timing") {
@@ -647,3 +681,27 @@ int hnswgpu_device_info(char* name, int len, int64_t* info) {
 }
 
 }  // extern "C"
+
+extern "C" int hnswgpu_get_row(hnswgpu_index* h, uint32_t id, int storage, void* out, float* scale_norm) {
+  return guarded([&] {
+    if (h == nullptr || out == nullptr) throw std::invalid_argument("get_row: null pointer");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (id >= h->view.count) throw std::out_of_range("get_row: id out of range");
+    const size_t p = h->view.padded;
+    if (storage == 0) {
+      check(hipMemcpy(out, h->view.vecs + id * p, p * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy row");
+      return;
+    }
+    if (storage != h->view.storage) throw std::invalid_argument("get_row: that storage is not built");
+    const size_t width = storage == 1 ? 2 : 1;
+    check(hipMemcpy(out, static_cast<const uint8_t*>(h->view.vlow) + id * p * width, p * width,
+                    hipMemcpyDeviceToHost), "hipMemcpy row");
+    if (scale_norm != nullptr) {
+      float sn[2] = {1.0f, 0.0f};
+      if (storage == 2) check(hipMemcpy(&sn[0], h->view.vscale + id, sizeof(float), hipMemcpyDeviceToHost), "scale");
+      check(hipMemcpy(&sn[1], h->view.vnorm + id, sizeof(float), hipMemcpyDeviceToHost), "norm");
+      scale_norm[0] = sn[0];
+      scale_norm[1] = sn[1];
+    }
+  });
+}

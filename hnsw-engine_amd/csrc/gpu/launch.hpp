@@ -23,7 +23,16 @@ struct DevIndexView {
   int32_t max_level;
   int32_t metric;            // 0 = L2, 1 = inner product, 2 = cosine
   uint32_t has_dups;         // some adjacency list repeats an id
+  int32_t storage;           // traversal copy: 0 = the fp32 rows, 1 = bf16, 2 = fp8 e4m3 + scale
+  const void* vlow;          // count x padded bf16 or fp8 (storage != 0)
+  const float* vscale;       // count fp8 scales (storage == 2)
+  const float* vnorm;        // count |x^|^2 of the stored rows (storage != 0)
 };
+
+/// Per-wavefront LDS words of the search kernels for an LDS hash of 2^log2h slots.
+__host__ __device__ inline uint32_t lds_words_per_wave(uint32_t log2h, uint32_t padded) {
+  return (1u << log2h) + 192u + padded + padded / 2u + 4u;
+}
 
 struct OutRow {
   uint64_t label;
@@ -64,6 +73,10 @@ struct SearchRequest {
   int force_spill;           // route every query through the general kernel
 };
 
+/// Build the reduced copy of the rows for `storage` (1 = bf16, 2 = fp8) into
+/// vlow / vscale / vnorm (device buffers sized by the caller).
+hipError_t launch_build_storage(const DevIndexView& ix, int storage, void* vlow, float* vscale, float* vnorm,
+                                hipStream_t st);
 /// Prepare queries (copy into padded rows; unit-normalise for cosine).
 hipError_t launch_prepare(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st);
 /// Reset counters, run the fast kernel, then the general kernel on overflow.

@@ -1,0 +1,99 @@
+#pragma once
+
+// Reduced-precision copies of the resident vectors and their distances.
+//
+//   BF16  2 bytes per element, round-to-nearest-even from fp32.
+//   FP8   OCP e4m3 (gfx950's native fp8 format, largest finite 448), one byte
+//         per element, with a per-vector fp32 scale s = max|x| / 448:
+//         x ~ s * e4m3(x / s).
+// Both keep |x^|^2 per row (of the stored, rounded vector) so that L2 can be
+// evaluated as |q^|^2 + |x^|^2 - 2 q^.x^. Dot products accumulate in fp32.
+// These distances only steer the traversal; the final candidates are re-ranked
+// with the exact fp32 distance, so reported distances are exact.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace hnswamd {
+namespace gpu {
+
+enum Storage : int { kStoreF32 = 0, kStoreBF16 = 1, kStoreFP8 = 2 };
+
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t bf16_bits_rne(float f) {
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u);  // inf / nan
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+__device__ __forceinline__ float bf16_value(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
+
+__device__ __forceinline__ float dot2_bf16(uint32_t a, uint32_t b, float c) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2_t, a), __builtin_bit_cast(bf16x2_t, b), c,
+                                         false);
+}
+
+/// Dot product of a bf16 row with the bf16 query (both `padded` elements),
+/// by the four lanes of a quad: lane g takes 16-byte pieces g, g+4, ...
+/// Identical in all four lanes.
+__device__ __forceinline__ float quad_dot_bf16(const uint16_t* __restrict__ row, const uint16_t* q,
+                                               uint32_t padded, bool active) {
+  const uint32_t g = static_cast<uint32_t>(lane_id() & 3);
+  float a0 = 0.0f, a1 = 0.0f;
+  if (active) {
+    const uint4* r4 = reinterpret_cast<const uint4*>(row);
+    const uint4* q4 = reinterpret_cast<const uint4*>(q);
+    const uint32_t pieces = padded >> 3;
+#pragma unroll 4
+    for (uint32_t i = g; i < pieces; i += 4) {
+      const uint4 x = r4[i];
+      const uint4 y = q4[i];
+      a0 = dot2_bf16(x.x, y.x, a0);
+      a1 = dot2_bf16(x.y, y.y, a1);
+      a0 = dot2_bf16(x.z, y.z, a0);
+      a1 = dot2_bf16(x.w, y.w, a1);
+    }
+  }
+  float v = a0 + a1;
+  v = v + quad_xor1(v);
+  v = v + quad_xor2(v);
+  return v;
+}
+
+/// Dot product of an e4m3 row (before its scale) with the fp32 query.
+__device__ __forceinline__ float quad_dot_fp8(const uint8_t* __restrict__ row, const float* q, uint32_t padded,
+                                              bool active) {
+  const uint32_t g = static_cast<uint32_t>(lane_id() & 3);
+  float a0 = 0.0f, a1 = 0.0f;
+  if (active) {
+    const uint4* r4 = reinterpret_cast<const uint4*>(row);
+    const uint32_t pieces = padded >> 4;  // 16 elements per piece
+#pragma unroll 2
+    for (uint32_t i = g; i < pieces; i += 4) {
+      const uint4 x = r4[i];
+      const float4* qp = reinterpret_cast<const float4*>(q + 16 * i);
+      const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float4 y = qp[j];
+        const f32x2_t lo = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(w[j]), false);
+        const f32x2_t hi = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(w[j]), true);
+        a0 = __builtin_fmaf(lo.x, y.x, a0);
+        a1 = __builtin_fmaf(lo.y, y.y, a1);
+        a0 = __builtin_fmaf(hi.x, y.z, a0);
+        a1 = __builtin_fmaf(hi.y, y.w, a1);
+      }
+    }
+  }
+  float v = a0 + a1;
+  v = v + quad_xor1(v);
+  v = v + quad_xor2(v);
+  return v;
+}
+
+}  // namespace gpu
+}  // namespace hnswamd

@@ -33,6 +33,7 @@
 #include "device_common.hpp"
 #include "launch.hpp"
 #include "quad_distance.hpp"
+#include "reduced.hpp"
 
 namespace hnswamd {
 namespace gpu {
@@ -62,20 +63,48 @@ struct WaveLds {
   uint32_t* nd;    // 64 ordered distances
   uint32_t* nel;   // 64 eligibility flags
   float* q;        // the prepared query (padded)
+  uint16_t* qb;    // the query rounded to bf16 (bf16 storage)
+  float* qn;       // |q^|^2 of the query as the reduced distance sees it
 };
 
-/// Ordered distances of the rows w.nid[0..U) into w.nd[0..U), 16 rows per step.
-template <int PC>
+__device__ __forceinline__ WaveLds wave_lds(uint32_t* base, uint32_t log2h, uint32_t padded) {
+  const uint32_t H = 1u << log2h;
+  return WaveLds{base,
+                 base + H,
+                 base + H + 64,
+                 base + H + 128,
+                 reinterpret_cast<float*>(base + H + 192),
+                 reinterpret_cast<uint16_t*>(base + H + 192 + padded),
+                 reinterpret_cast<float*>(base + H + 192 + padded + padded / 2)};
+}
+
+/// Ordered distances of the rows w.nid[0..U) into w.nd[0..U), 16 rows per step,
+/// from the fp32 rows (ST = 0, exact) or the bf16 / fp8 copy (ST = 1 / 2).
+template <int PC, int ST = kStoreF32>
 __device__ __forceinline__ void row_dists(const DevIndexView& ix, const WaveLds& w, uint32_t U) {
   const int lane = lane_id();
   const bool l2 = ix.metric == 0;
+  const float qn = ST == kStoreF32 ? 0.0f : w.qn[0];
   for (uint32_t base = 0; base < U; base += 16) {
     const uint32_t r = base + static_cast<uint32_t>(lane >> 2);
     const bool active = r < U;
     const uint32_t id = active ? w.nid[r] : 0u;
-    const float* row = ix.vecs + static_cast<size_t>(id) * ix.padded;
-    float d = quad_raw<PC>(l2, row, w.q, ix.padded >> 4, false, active);
-    if (!l2) d = 1.0f - d;
+    float d;
+    if constexpr (ST == kStoreF32) {
+      const float* row = ix.vecs + static_cast<size_t>(id) * ix.padded;
+      d = quad_raw<PC>(l2, row, w.q, ix.padded >> 4, false, active);
+      if (!l2) d = 1.0f - d;
+    } else {
+      float dp;
+      if constexpr (ST == kStoreBF16) {
+        const uint16_t* row = static_cast<const uint16_t*>(ix.vlow) + static_cast<size_t>(id) * ix.padded;
+        dp = quad_dot_bf16(row, w.qb, ix.padded, active);
+      } else {
+        const uint8_t* row = static_cast<const uint8_t*>(ix.vlow) + static_cast<size_t>(id) * ix.padded;
+        dp = quad_dot_fp8(row, w.q, ix.padded, active) * (active ? ix.vscale[id] : 0.0f);
+      }
+      d = l2 ? (qn + (active ? ix.vnorm[id] : 0.0f)) - 2.0f * dp : 1.0f - dp;
+    }
     if (active && (lane & 3) == 0) w.nd[r] = ord_of(d);
   }
 }
@@ -166,7 +195,7 @@ struct Work {
 };
 
 /// Greedy descent from `cur` through layers max_level..1.
-template <int PC>
+template <int PC, int ST>
 __device__ uint64_t descend(const DevIndexView& ix, const WaveLds& w, uint64_t cur, Work& work) {
   const int lane = lane_id();
   for (int L = ix.max_level; L > 0; --L) {
@@ -179,7 +208,7 @@ __device__ uint64_t descend(const DevIndexView& ix, const WaveLds& w, uint64_t c
       wave_sync();
       if (static_cast<uint32_t>(lane) < cnt) w.nid[lane] = l[1 + lane];
       wave_sync();
-      row_dists<PC>(ix, w, cnt);
+      row_dists<PC, ST>(ix, w, cnt);
       wave_sync();
       const uint64_t mine = static_cast<uint32_t>(lane) < cnt ? key_of(w.nd[lane], static_cast<uint32_t>(lane))
                                                               : kKeyInf;
@@ -192,20 +221,58 @@ __device__ uint64_t descend(const DevIndexView& ix, const WaveLds& w, uint64_t c
 }
 
 /// Load the prepared query into LDS and compute the entry point distance.
-template <int PC>
+template <int PC, int ST = kStoreF32>
 __device__ uint64_t start_query(const DevIndexView& ix, const SearchScratch& s, const WaveLds& w, uint32_t qi,
                                 Work& work) {
   const int lane = lane_id();
   const float* src = s.qprep + static_cast<size_t>(qi) * ix.padded;
   wave_sync();
-  for (uint32_t i = static_cast<uint32_t>(lane); i < ix.padded; i += 64) w.q[i] = src[i];
+  float sq = 0.0f;
+  for (uint32_t i = static_cast<uint32_t>(lane); i < ix.padded; i += 64) {
+    const float x = src[i];
+    w.q[i] = x;
+    if constexpr (ST == kStoreBF16) {
+      const uint32_t b = bf16_bits_rne(x);
+      w.qb[i] = static_cast<uint16_t>(b);
+      const float xb = bf16_value(b);
+      sq = __builtin_fmaf(xb, xb, sq);
+    } else if constexpr (ST == kStoreFP8) {
+      sq = __builtin_fmaf(x, x, sq);
+    }
+  }
+  if constexpr (ST != kStoreF32) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) sq += __shfl_xor(sq, o);
+    if (lane == 0) w.qn[0] = sq;
+  }
   if (lane == 0) w.nid[0] = ix.entry;
   wave_sync();
-  row_dists<PC>(ix, w, 1);
+  row_dists<PC, ST>(ix, w, 1);
   wave_sync();
   const uint64_t cur = key_of(uniform(w.nd[0]), ix.entry);
   work.dist += 1;
-  return descend<PC>(ix, w, cur, work);
+  return descend<PC, ST>(ix, w, cur, work);
+}
+
+/// Re-rank R (sR approximate keys) with exact fp32 distances, in place.
+template <int ER, int PC>
+__device__ void rerank_exact(const DevIndexView& ix, const WaveLds& w, WaveSorted<ER>& R, uint32_t sR) {
+  const int lane = lane_id();
+  WaveSorted<ER> X;
+  X.clear();
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    const uint32_t base = static_cast<uint32_t>(e) * 64u;
+    if (base >= sR) break;
+    const uint32_t cnt = sR - base < 64u ? sR - base : 64u;
+    wave_sync();
+    if (static_cast<uint32_t>(lane) < cnt) w.nid[lane] = key_id(R.v[e]);
+    wave_sync();
+    row_dists<PC, kStoreF32>(ix, w, cnt);
+    wave_sync();
+    for (uint32_t u = 0; u < cnt; ++u) X.insert(key_of(uniform(w.nd[u]), uniform(w.nid[u])), sR);
+  }
+  R = X;
 }
 
 /// A node's layer-0 list in registers: lane j holds entry j + 1, `cnt` the
@@ -316,7 +383,7 @@ __device__ __forceinline__ uint32_t first_unexpanded(const uint32_t (&fl)[ER], u
 /// expanded again, so popping "first unexpanded slot of R, else the smallest
 /// tie" is the exact Alg. 2 order, and the loop ends exactly when Alg. 2 would
 /// stop (the smallest remaining candidate is further than the worst result).
-template <int ER, int PC>
+template <int ER, int PC, int ST>
 __device__ Handoff plain_query(const DevIndexView& ix, const SearchRequest& r, const SearchScratch& s,
                                const WaveLds& w, Visited& vis, uint32_t qi) {
   const int lane = lane_id();
@@ -326,7 +393,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const SearchRequest& r, c
   vis.in_lds = 0;
   vis.in_hbm = 0;
   Work work;
-  const uint64_t ep = start_query<PC>(ix, s, w, qi, work);
+  const uint64_t ep = start_query<PC, ST>(ix, s, w, qi, work);
 
   WaveSorted<ER> R;
   R.clear();
@@ -373,7 +440,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const SearchRequest& r, c
     vis.count(U);
     work.dist += U;
     ++work.exp0;
-    row_dists<PC>(ix, w, U);
+    row_dists<PC, ST>(ix, w, U);
     wave_sync();
     const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
     const uint32_t id_u = w.nid[lane];
@@ -413,6 +480,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const SearchRequest& r, c
   if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
   vis.finish();
   if (result != Handoff::kNone) return result;
+  if constexpr (ST != kStoreF32) rerank_exact<ER, PC>(ix, w, R, sR);
   write_results<ER>(ix, r, R, sR, work, qi);
   return Handoff::kNone;
 }
@@ -422,7 +490,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const SearchRequest& r, c
 /// smallest entry it ever dropped is remembered -- the search stays exact as
 /// long as that entry is never the smallest live candidate, otherwise the
 /// query goes to the general kernel.
-template <int ER, int EC, int PC>
+template <int ER, int EC, int PC, int ST>
 __device__ Handoff filtered_query(const DevIndexView& ix, const SearchRequest& r, const SearchScratch& s,
                                   const WaveLds& w, Visited& vis, uint32_t qi) {
   const int lane = lane_id();
@@ -433,7 +501,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const SearchRequest& r
   vis.in_lds = 0;
   vis.in_hbm = 0;
   Work work;
-  const uint64_t ep = start_query<PC>(ix, s, w, qi, work);
+  const uint64_t ep = start_query<PC, ST>(ix, s, w, qi, work);
 
   WaveSorted<ER> R;
   WaveSorted<EC> C;
@@ -479,7 +547,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const SearchRequest& r
     vis.count(U);
     work.dist += U;
     ++work.exp0;
-    row_dists<PC>(ix, w, U);
+    row_dists<PC, ST>(ix, w, U);
     wave_sync();
     const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
     const uint32_t id_u = w.nid[lane];
@@ -507,19 +575,19 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const SearchRequest& r
   if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
   vis.finish();
   if (result != Handoff::kNone) return result;
+  if constexpr (ST != kStoreF32) rerank_exact<ER, PC>(ix, w, R, sR);
   write_results<ER>(ix, r, R, sR, work, qi);
   return Handoff::kNone;
 }
 
-template <int ER, int EC, int PC, bool kFiltered>
+template <int ER, int EC, int PC, bool kFiltered, int ST>
 __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView ix, SearchRequest r,
                                                                    SearchScratch s) {
   extern __shared__ uint32_t lds[];
   const int lane = lane_id();
-  const uint32_t H = 1u << r.hash_log2;
-  const uint32_t per_wave = H + 192u + ix.padded;
+  const uint32_t per_wave = lds_words_per_wave(r.hash_log2, ix.padded);
   uint32_t* base = lds + static_cast<size_t>(threadIdx.x >> 6) * per_wave;
-  const WaveLds w{base, base + H, base + H + 64, base + H + 128, reinterpret_cast<float*>(base + H + 192)};
+  const WaveLds w = wave_lds(base, r.hash_log2, ix.padded);
   const size_t wave_slot = static_cast<size_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   Visited vis{w.hash, r.hash_log2,
               s.visit_tables != nullptr ? s.visit_tables + (wave_slot << s.visit_log2) : nullptr,
@@ -532,9 +600,9 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView 
     Handoff h = Handoff::kCandidates;
     if (!r.force_spill) {
       if constexpr (kFiltered) {
-        h = filtered_query<ER, EC, PC>(ix, r, s, w, vis, qi);
+        h = filtered_query<ER, EC, PC, ST>(ix, r, s, w, vis, qi);
       } else {
-        h = plain_query<ER, PC>(ix, r, s, w, vis, qi);
+        h = plain_query<ER, PC, ST>(ix, r, s, w, vis, qi);
       }
     }
     if (h != Handoff::kNone && lane == 0) {
@@ -588,9 +656,15 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) general_search(DevIndexVi
   const int lane = lane_id();
   const uint32_t wave_slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave_slot >= s.spill_waves) return;
-  const uint32_t per_wave = 192u + ix.padded;
+  const uint32_t per_wave = lds_words_per_wave(0, ix.padded) - 1u;  // no hash
   uint32_t* base = lds + static_cast<size_t>(threadIdx.x >> 6) * per_wave;
-  const WaveLds w{nullptr, base, base + 64, base + 128, reinterpret_cast<float*>(base + 192)};
+  const WaveLds w{nullptr,
+                  base,
+                  base + 64,
+                  base + 128,
+                  reinterpret_cast<float*>(base + 192),
+                  reinterpret_cast<uint16_t*>(base + 192 + ix.padded),
+                  reinterpret_cast<float*>(base + 192 + ix.padded + ix.padded / 2)};
   uint32_t* bits = s.spill_words + static_cast<size_t>(wave_slot) * s.spill_bitmap_words;
   uint64_t* Cheap = s.spill_heaps + static_cast<size_t>(wave_slot) * s.spill_heap_stride;
   uint64_t* Rheap = Cheap + (ix.count + 1u);
@@ -730,36 +804,93 @@ __global__ void build_eligibility(uint32_t count, const uint64_t* labels, const 
   elig[word] = out;
 }
 
-template <int ER, int EC, int PC, bool kFiltered>
+template <int ER, int EC, int PC, bool kFiltered, int ST>
 hipError_t run_fast(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
-  const size_t lds = static_cast<size_t>(r.waves_per_block) * ((1u << r.hash_log2) + 192u + ix.padded) * 4u;
+  const size_t lds = static_cast<size_t>(r.waves_per_block) * lds_words_per_wave(r.hash_log2, ix.padded) * 4u;
   static bool configured = false;
   if (!configured) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fast_search<ER, EC, PC, kFiltered>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fast_search<ER, EC, PC, kFiltered, ST>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
     configured = true;
   }
-  hipLaunchKernelGGL((fast_search<ER, EC, PC, kFiltered>), dim3(r.blocks), dim3(r.waves_per_block * 64), lds, st,
-                     ix, r, s);
+  hipLaunchKernelGGL((fast_search<ER, EC, PC, kFiltered, ST>), dim3(r.blocks), dim3(r.waves_per_block * 64), lds,
+                     st, ix, r, s);
   return hipGetLastError();
 }
 
-template <int PC, bool kFiltered>
+template <int PC, bool kFiltered, int ST>
 hipError_t pick_capacity(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
-  if (r.ef <= 64) return run_fast<1, 2, PC, kFiltered>(ix, r, s, st);
-  if (r.ef <= 128) return run_fast<2, 4, PC, kFiltered>(ix, r, s, st);
-  if (r.ef <= 256) return run_fast<4, 8, PC, kFiltered>(ix, r, s, st);
-  return run_fast<8, 16, PC, kFiltered>(ix, r, s, st);
+  if (r.ef <= 64) return run_fast<1, 2, PC, kFiltered, ST>(ix, r, s, st);
+  if (r.ef <= 128) return run_fast<2, 4, PC, kFiltered, ST>(ix, r, s, st);
+  if (r.ef <= 256) return run_fast<4, 8, PC, kFiltered, ST>(ix, r, s, st);
+  return run_fast<8, 16, PC, kFiltered, ST>(ix, r, s, st);
+}
+
+template <int PC, int ST>
+hipError_t pick_filter(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
+  if (r.elig != nullptr) return pick_capacity<PC, true, ST>(ix, r, s, st);
+  return pick_capacity<PC, false, ST>(ix, r, s, st);
 }
 
 template <int PC>
 hipError_t pick_variant(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
-  if (r.elig != nullptr) return pick_capacity<PC, true>(ix, r, s, st);
-  return pick_capacity<PC, false>(ix, r, s, st);
+  if (ix.storage == kStoreBF16) return pick_filter<PC, kStoreBF16>(ix, r, s, st);
+  if (ix.storage == kStoreFP8) return pick_filter<PC, kStoreFP8>(ix, r, s, st);
+  return pick_filter<PC, kStoreF32>(ix, r, s, st);
+}
+
+/// One wavefront per row: the reduced copy of an fp32 row and |x^|^2 of it.
+__global__ void __launch_bounds__(256) build_storage(DevIndexView ix, int storage, void* vlow, float* vscale,
+                                                     float* vnorm) {
+  const int lane = lane_id();
+  const uint32_t id = blockIdx.x * 4u + (threadIdx.x >> 6);
+  if (id >= ix.count) return;
+  const float* x = ix.vecs + static_cast<size_t>(id) * ix.padded;
+  float sq = 0.0f;
+  if (storage == kStoreBF16) {
+    uint16_t* out = static_cast<uint16_t*>(vlow) + static_cast<size_t>(id) * ix.padded;
+    for (uint32_t i = static_cast<uint32_t>(lane); i < ix.padded; i += 64) {
+      const uint32_t b = bf16_bits_rne(x[i]);
+      out[i] = static_cast<uint16_t>(b);
+      const float v = bf16_value(b);
+      sq = __builtin_fmaf(v, v, sq);
+    }
+  } else {
+    float m = 0.0f;
+    for (uint32_t i = static_cast<uint32_t>(lane); i < ix.padded; i += 64) m = fmaxf(m, fabsf(x[i]));
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    const float scale = m > 0.0f ? m / 448.0f : 1.0f;
+    const float inv = m > 0.0f ? 448.0f / m : 1.0f;
+    uint8_t* out = static_cast<uint8_t*>(vlow) + static_cast<size_t>(id) * ix.padded;
+    // Lane pairs of elements: lane l encodes elements 2l, 2l+1 of each 128-wide slice.
+    for (uint32_t i = 2u * static_cast<uint32_t>(lane); i < ix.padded; i += 128) {
+      const float a = fminf(fmaxf(x[i] * inv, -448.0f), 448.0f);
+      const float b = fminf(fmaxf(x[i + 1] * inv, -448.0f), 448.0f);
+      const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+      out[i] = static_cast<uint8_t>(packed & 0xFF);
+      out[i + 1] = static_cast<uint8_t>((packed >> 8) & 0xFF);
+      const f32x2_t back = __builtin_amdgcn_cvt_pk_f32_fp8(packed, false);
+      const float va = back.x * scale, vb = back.y * scale;
+      sq = __builtin_fmaf(va, va, sq);
+      sq = __builtin_fmaf(vb, vb, sq);
+    }
+    if (lane == 0) vscale[id] = scale;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) sq += __shfl_xor(sq, o);
+  if (lane == 0) vnorm[id] = sq;
 }
 
 }  // namespace
+
+hipError_t launch_build_storage(const DevIndexView& ix, int storage, void* vlow, float* vscale, float* vnorm,
+                                hipStream_t st) {
+  if (ix.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(build_storage, dim3((ix.count + 3) / 4), dim3(256), 0, st, ix, storage, vlow, vscale, vnorm);
+  return hipGetLastError();
+}
 
 hipError_t launch_prepare(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
   if (r.nq == 0) return hipSuccess;
@@ -798,7 +929,7 @@ hipError_t launch_search(const DevIndexView& ix, const SearchRequest& r, SearchS
     e = hipEventRecord(fast_end, st);
     if (e != hipSuccess) return e;
   }
-  const size_t lds = static_cast<size_t>(kWavesPerBlock) * (192u + ix.padded) * 4u;
+  const size_t lds = static_cast<size_t>(kWavesPerBlock) * (lds_words_per_wave(0, ix.padded) - 1u) * 4u;
   const uint32_t blocks = (s.spill_waves + kWavesPerBlock - 1) / kWavesPerBlock;
   if (blocks == 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(general_search, dim3(blocks), dim3(kWavesPerBlock * 64), lds, st, ix, rr, s);

@@ -45,6 +45,7 @@ def _load():
         "hnswgpu_set_option": ([p, C.c_char_p, i64], C.c_int),
         "hnswgpu_get_option": ([p, C.c_char_p, p], C.c_int),
         "hnswgpu_read_stats": ([p, p, u64], C.c_int),
+        "hnswgpu_get_row": ([p, C.c_uint32, C.c_int, p, p], C.c_int),
         "hnswgpu_malloc": ([u64, p], C.c_int),
         "hnswgpu_mfree": ([p], C.c_int),
         "hnswgpu_memcpy": ([p, p, u64, C.c_int], C.c_int),
@@ -235,6 +236,14 @@ class DeviceIndex:
         w, size = filter_words(filter)
         _check(_lib.hnswgpu_search_batch(self._h, _ptr(q), nq, k, ef, _ptr(w), size, _ptr(out), _ptr(counts), 0))
         return out.reshape(nq, k), counts
+
+    def get_row(self, node_id: int, storage: int, padded_dim: int):
+        """The stored row of an internal id: (values, scale, |x^|^2)."""
+        dtype = {0: np.float32, 1: np.uint16, 2: np.uint8}[storage]
+        out = np.empty(padded_dim, dtype=dtype)
+        sn = np.zeros(2, np.float32)
+        _check(_lib.hnswgpu_get_row(self._h, node_id, storage, _ptr(out), _ptr(sn)))
+        return out, float(sn[0]), float(sn[1])
 
     def read_stats(self, nq: int):
         out = np.empty((nq, 3), dtype=np.uint32)

@@ -119,6 +119,16 @@ int hnswgpu_search_batch_device(hnswgpu_index* h, const float* d_qs, uint64_t nq
 int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value);
 int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value);
 
+/* Precision of the traversal (no reference counterpart): option "storage" =
+ * 0 traverses the fp32 rows (exact ids); 1 keeps a bf16 copy of the rows and
+ * 2 an OCP fp8 e4m3 copy with a per-vector fp32 scale (max|x| / 448), both
+ * with |x^|^2 per row. Distances then accumulate in fp32 over the copy and the
+ * final ef candidates are re-ranked with exact fp32 distances, so every
+ * returned distance is exact; ids may differ from the fp32 traversal.
+ * hnswgpu_get_row copies one stored row (storage 0: padded_dim fp32, 1:
+ * padded_dim bf16, 2: padded_dim fp8 bytes) and, for 1 / 2, {scale, |x^|^2}. */
+int hnswgpu_get_row(hnswgpu_index* h, uint32_t id, int storage, void* out, float* scale_norm);
+
 /* Per-query work of the last search (collect_stats on): nq x 3 of
  * {distances computed, layer-0 lists read, upper-layer lists read}, the
  * counts from which algorithmic HBM bytes follow. No reference counterpart. */
