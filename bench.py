@@ -44,17 +44,38 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def ground_truth(X: np.ndarray, Q: np.ndarray, k: int) -> tuple[np.ndarray, float]:
+MFMA_FP32_PEAK_TFLOPS = 157.3
+
+
+def ground_truth(X: np.ndarray, Q: np.ndarray, k: int) -> tuple[np.ndarray, float, dict]:
     """Exact top-k labels by the engine's GPU brute force (best k by (distance,
     label)). The data are integers in [0, 255], so every fp32 partial sum is an
     exact integer below 2^24 and the fp32 order equals the FP64 order; a sample
     of queries is re-checked in FP64 on the host. Returns (labels, fraction of
-    the sample whose FP64 top-k labels agree)."""
+    the sample whose FP64 top-k labels agree, brute-force timings: the MFMA
+    prefilter kernel's fp32 matrix rate over 2*nq*n*dim flops and the device
+    time of the whole search)."""
     import hnswgpu
 
     bf = hnswgpu.BruteForceIndex(X.shape[1], "l2")
     bf.add(X)
+    bf.set_option("kernel_timing", 1)
     out = bf.search_rows(Q, k)["label"].astype(np.int64)
+    gemm, total = [], []
+    for _ in range(3):
+        again = bf.search_rows(Q, k)["label"].astype(np.int64)
+        assert np.array_equal(again, out)
+        gemm.append(bf.get_option("last_gemm_ns") / 1e6)
+        total.append(bf.get_option("last_total_ns") / 1e6)
+    flops = 2.0 * Q.shape[0] * X.shape[0] * X.shape[1]
+    g, t = float(np.median(gemm)), float(np.median(total))
+    info = {"n": int(X.shape[0]), "nq": int(Q.shape[0]), "k": k, "mfma": bf.get_option("last_mfma"),
+            "chunks": bf.get_option("last_chunks"), "scan_requeries": bf.get_option("last_redo"),
+            "gemm_ms": round(g, 3), "device_ms": round(t, 3),
+            "qps": round(Q.shape[0] / (t / 1e3), 1) if t > 0 else None,
+            "mfma_tflops": round(flops / (g / 1e3) / 1e12, 2) if g > 0 else None,
+            "mfma_frac": round(flops / (g / 1e3) / 1e12 / MFMA_FP32_PEAK_TFLOPS, 4) if g > 0 else None,
+            "mfma_peak_tflops": MFMA_FP32_PEAK_TFLOPS}
     del bf
     agree = 0
     sample = min(32, Q.shape[0])
@@ -62,7 +83,7 @@ def ground_truth(X: np.ndarray, Q: np.ndarray, k: int) -> tuple[np.ndarray, floa
         d64 = ((X.astype(np.float64) - Q[i].astype(np.float64)) ** 2).sum(1)
         top = np.lexsort((np.arange(X.shape[0]), d64))[:k]
         agree += int(np.array_equal(top, out[i]))
-    return out, agree / sample
+    return out, agree / sample, info
 
 
 def recall_at(labels: np.ndarray, truth: np.ndarray, k: int) -> float:
@@ -172,8 +193,8 @@ def main() -> int:
         ix.save(index_path)
 
     t0 = time.perf_counter()
-    truth, truth_check = (None, None) if args.profile else ground_truth(X, Q, k)
-    log(f"ground truth in {time.perf_counter() - t0:.1f}s (FP64 check agreement {truth_check})")
+    truth, truth_check, brute = (None, None, None) if args.profile else ground_truth(X, Q, k)
+    log(f"ground truth in {time.perf_counter() - t0:.1f}s (FP64 check agreement {truth_check}); brute force {brute}")
 
     dev = ix.device()
     for kv in args.set:
@@ -302,6 +323,7 @@ def main() -> int:
                      "kernel": "fast_search", "at_ef": op["ef"], "kernel_ms": op["kernel_ms"],
                      "bytes_per_query": op["bytes_per_query"]},
         "cpu_baseline": cpu,
+        "brute_force": brute,
         **reduced,
         "latency_ms": {"single_query_p50": round(float(np.percentile(lat, 50)), 4),
                        "single_query_p99": round(float(np.percentile(lat, 99)), 4),

@@ -373,6 +373,7 @@ void BruteForceIndex::run(const float* qs, std::size_t nq, std::size_t k, Neighb
     dev_ = std::make_unique<Device>();
     dev_->set = hnswamd::gpu::brute_create(dim_, static_cast<int>(metric_), data_.data(), labels_.data(),
                                            labels_.size());
+    for (const auto& [name, value] : options_) hnswamd::gpu::brute_set_option(dev_->set, name, value);
   }
   std::vector<uint64_t> allow;
   const uint64_t* allow_ptr = nullptr;
@@ -384,6 +385,34 @@ void BruteForceIndex::run(const float* qs, std::size_t nq, std::size_t k, Neighb
     allow_ptr = allow.data();
   }
   hnswamd::gpu::brute_search(dev_->set, qs, nq, k, allow_ptr, reinterpret_cast<hnswgpu_neighbor*>(out), counts);
+}
+
+void BruteForceIndex::set_device_option(const std::string& name, int64_t value) {
+  if (name.rfind("last_", 0) == 0) throw std::invalid_argument("read-only brute-force option: " + name);
+  if (dev_) hnswamd::gpu::brute_set_option(dev_->set, name, value);
+  bool found = false;
+  for (auto& [n, v] : options_) {
+    if (n == name) {
+      v = value;
+      found = true;
+    }
+  }
+  if (!found) {
+    // Validate the name before keeping it for a device set created later.
+    if (name != "force_scan" && name != "kernel_timing" && name != "chunk_rows") {
+      throw std::invalid_argument("unknown brute-force option: " + name);
+    }
+    options_.emplace_back(name, value);
+  }
+}
+
+int64_t BruteForceIndex::device_option(const std::string& name) const {
+  if (dev_) return hnswamd::gpu::brute_get_option(dev_->set, name);
+  for (const auto& [n, v] : options_) {
+    if (n == name) return v;
+  }
+  if (name == "force_scan" || name == "kernel_timing" || name == "chunk_rows" || name.rfind("last_", 0) == 0) return 0;
+  throw std::invalid_argument("unknown brute-force option: " + name);
 }
 
 void BruteForceIndex::search_batch(const float* qs, std::size_t nq, std::size_t k, Neighbor* out,
@@ -544,6 +573,18 @@ int hnsw_brute_force_create(uint64_t dim, int32_t metric, hnsw_brute_force** out
 void hnsw_brute_force_free(hnsw_brute_force* h) { delete h; }
 int hnsw_brute_force_add(hnsw_brute_force* h, const float* vecs, const uint64_t* labels, uint64_t n) {
   return guarded([&] { h->bf.add(vecs, labels, n); });
+}
+int hnsw_brute_force_set_option(hnsw_brute_force* h, const char* name, int64_t value) {
+  return guarded([&] {
+    if (name == nullptr) throw std::invalid_argument("null option name");
+    h->bf.set_device_option(name, value);
+  });
+}
+int hnsw_brute_force_get_option(hnsw_brute_force* h, const char* name, int64_t* value) {
+  return guarded([&] {
+    if (name == nullptr || value == nullptr) throw std::invalid_argument("null pointer");
+    *value = h->bf.device_option(name);
+  });
 }
 int hnsw_brute_force_search_batch(hnsw_brute_force* h, const float* qs, uint64_t nq, uint64_t k,
                                   hnswgpu_neighbor* out, uint64_t threads, const uint64_t* fw, uint64_t fsize) {

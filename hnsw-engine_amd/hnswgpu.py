@@ -76,6 +76,8 @@ def _load():
         "hnsw_brute_force_free": ([p], None),
         "hnsw_brute_force_add": ([p, p, p, u64], C.c_int),
         "hnsw_brute_force_search_batch": ([p, p, u64, u64, p, u64, p, u64], C.c_int),
+        "hnsw_brute_force_set_option": ([p, C.c_char_p, C.c_int64], C.c_int),
+        "hnsw_brute_force_get_option": ([p, C.c_char_p, p], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -395,3 +397,11 @@ class BruteForceIndex:
         w, size = filter_words(filter)
         _check(_lib.hnsw_brute_force_search_batch(self._h, _ptr(q), q.shape[0], k, _ptr(out), 0, _ptr(w), size))
         return out.reshape(q.shape[0], k)
+
+    def set_option(self, name: str, value: int) -> None:
+        _check(_lib.hnsw_brute_force_set_option(self._h, name.encode(), int(value)))
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int64()
+        _check(_lib.hnsw_brute_force_get_option(self._h, name.encode(), C.byref(v)))
+        return v.value

@@ -102,6 +102,12 @@ class BruteForceIndex {
   std::size_t size() const noexcept { return labels_.size(); }
   std::size_t dim() const noexcept { return dim_; }
 
+  /// GPU options (no counterpart in the reference): force_scan, kernel_timing,
+  /// chunk_rows; read-only last_mfma, last_redo, last_chunks, last_gemm_ns,
+  /// last_total_ns describe the last search.
+  void set_device_option(const std::string& name, int64_t value);
+  int64_t device_option(const std::string& name) const;
+
   struct Device;
 
  private:
@@ -111,6 +117,7 @@ class BruteForceIndex {
   Metric metric_;
   std::vector<float> data_;
   std::vector<uint64_t> labels_;
+  std::vector<std::pair<std::string, int64_t>> options_;
   mutable std::unique_ptr<Device> dev_;
 };
 

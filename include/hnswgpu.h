@@ -204,6 +204,17 @@ int hnsw_brute_force_add(hnsw_brute_force* h, const float* vecs, const uint64_t*
 int hnsw_brute_force_search_batch(hnsw_brute_force* h, const float* qs, uint64_t nq, uint64_t k,
                                   hnswgpu_neighbor* out, uint64_t num_threads, const uint64_t* filter_words,
                                   uint64_t filter_size);
+/* GPU options of a BruteForceIndex; no counterpart in the reference.
+ *   force_scan     1: always use the exact scan kernel (no MFMA prefilter)
+ *   kernel_timing  1: time the device part of each search with HIP events
+ *   chunk_rows     rows per MFMA row chunk (0: chosen per call)
+ * read-only, about the last search:
+ *   last_mfma      1 if the MFMA prefilter ran (k <= 32, padded dim <= 256)
+ *   last_redo      queries it could not certify, re-run by the scan
+ *   last_chunks    row chunks of the MFMA grid
+ *   last_gemm_ns   MFMA kernel time; last_total_ns: device time of the search */
+int hnsw_brute_force_set_option(hnsw_brute_force* h, const char* name, int64_t value);
+int hnsw_brute_force_get_option(hnsw_brute_force* h, const char* name, int64_t* value);
 
 #ifdef __cplusplus
 }
