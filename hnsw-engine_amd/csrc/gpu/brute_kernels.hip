@@ -55,6 +55,7 @@ struct BruteSet {
   int force_scan = 0;
   int kernel_timing = 0;
   uint32_t chunk_rows = 0;       // 0: chosen per call
+  int mfma_variant = 1;          // 0: register-staged tiles, 1: global_load_lds tiles
   // last call
   int last_mfma = 0;
   uint32_t last_redo = 0;
@@ -213,7 +214,7 @@ __device__ __forceinline__ void lds_fence() {
 }
 
 template <int P16>
-__global__ void __launch_bounds__(512) brute_mfma(MfmaArgs a) {
+__global__ void __launch_bounds__(512, P16 <= 8 ? 4 : 2) brute_mfma(MfmaArgs a) {
   constexpr uint32_t P = 16u * P16;
   constexpr uint32_t S = P + 4u;                     // LDS row stride (floats)
   constexpr uint32_t kLoads = (kTileRows * P / 4 + 511) / 512;  // float4 per thread per tile
@@ -323,6 +324,156 @@ __global__ void __launch_bounds__(512) brute_mfma(MfmaArgs a) {
     __syncthreads();
     if (more) stage();
     __syncthreads();
+  }
+  if (q < a.nq) {
+    uint64_t* dst = a.cand + (static_cast<size_t>(q) * a.nchunks + chunk) * kChunkList + kLaneKeep * g;
+#pragma unroll
+    for (int s = 0; s < static_cast<int>(kLaneKeep); ++s) dst[s] = key_of(lo[s], li[s]);
+  }
+}
+
+/// Column piece stored in LDS slot s of tile row r (and the slot holding
+/// column piece s): 16-byte pieces are XOR-swizzled inside complete groups of
+/// eight, so the wavefront's 16-byte reads of one piece over 8 rows hit 8
+/// different bank groups.
+template <uint32_t PP>
+__device__ __forceinline__ uint32_t swz(uint32_t s, uint32_t r) {
+  return (s | 7u) < PP ? (s ^ (r & 7u)) : s;
+}
+
+/// As brute_mfma, with the row tiles copied HBM -> LDS by global_load_lds
+/// (no register staging) into two LDS buffers: the copy of tile t + 1 runs
+/// while tile t is multiplied.
+template <int P16>
+__global__ void __launch_bounds__(512, P16 <= 8 ? 4 : 2) brute_mfma_dma(MfmaArgs a) {
+  constexpr uint32_t P = 16u * P16;
+  constexpr uint32_t PP = P / 4;  // 16-byte pieces per row; one wave instruction moves 64 pieces
+  constexpr uint32_t kIssues = (PP + kMfmaWaves - 1) / kMfmaWaves;
+  // Two distinct LDS objects (not one array of two), so that the compiler
+  // sees that reads of one buffer do not depend on copies into the other and
+  // does not wait for those copies before the reads.
+  __shared__ float4 xsA[kTileRows * PP];
+  __shared__ float4 xsB[kTileRows * PP];
+  __shared__ float xnA[kTileRows];
+  __shared__ float xnB[kTileRows];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t j = lane & 15, g = lane >> 4;
+  const uint32_t qb = blockIdx.x % a.qblocks, chunk = blockIdx.x / a.qblocks;
+  const uint32_t q = qb * kMfmaQueries + 16u * wave + j;
+  const uint32_t r0 = chunk * a.rows_per_chunk;
+  const uint32_t r1 = min(a.n, r0 + a.rows_per_chunk);
+
+  float4 bq[P16];
+#pragma unroll
+  for (int b = 0; b < P16; ++b) {
+    bq[b] = q < a.nq ? *reinterpret_cast<const float4*>(a.qprep + static_cast<size_t>(q) * P + 16 * b + 4 * g)
+                     : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  uint32_t lo[kLaneKeep], li[kLaneKeep];
+#pragma unroll
+  for (int s = 0; s < static_cast<int>(kLaneKeep); ++s) {
+    lo[s] = kOrdInf;
+    li[s] = 0;
+  }
+  const bool l2 = a.metric == 0;
+
+  // Rows past the chunk end read the chunk's last row; the epilogue masks them.
+  // Addresses are 32-bit byte offsets from the chunk's first row.
+  const char* cbase = reinterpret_cast<const char*>(a.rows + static_cast<size_t>(r0) * P);
+  const uint32_t last = r1 - 1u - r0;
+  auto issue = [&](uint32_t rb, float4* dst) {
+    // The lane index is re-read here so that the per-lane offsets are
+    // recomputed at each copy rather than held in registers across the loop.
+    uint32_t ln = lane;
+    asm volatile("" : "+v"(ln));
+#pragma unroll
+    for (uint32_t i = 0; i < kIssues; ++i) {
+      const uint32_t inst = i * kMfmaWaves + wave;
+      if (inst < PP) {
+        const uint32_t p = inst * 64u + ln;
+        const uint32_t r = p / PP, c = swz<PP>(p % PP, r);
+        const uint32_t row = min(rb - r0 + r, last);
+        const uint32_t off = (row * P + 4u * c) * 4u;
+        typedef const __attribute__((address_space(1))) float* gptr_t;
+        typedef __attribute__((address_space(3))) float* lptr_t;
+        __builtin_amdgcn_global_load_lds((gptr_t)(cbase + off), (lptr_t)(dst + inst * 64u), 16, 0, 0);
+      }
+    }
+  };
+  // Next tile's norms (wave 0, one per lane) and allow word, loaded with its copy.
+  float xreg = 0.0f;
+  uint64_t wreg = ~0ULL;
+  auto side = [&](uint32_t rb) {
+    if (wave == 0) xreg = rb + lane < r1 ? a.xnorm[rb + lane] : 0.0f;
+    if (a.allow != nullptr) wreg = a.allow[rb >> 6];
+  };
+  // One tile: wait for its copy, start the next tile's copy into the other
+  // buffer, multiply, and keep the 8 best per lane.
+  const uint32_t slot0 = g ^ (j & 7u), slot1 = (4u + g) ^ (j & 7u);
+  auto tile = [&](uint32_t rb, const float4* X, float* xn, float4* Xnext, float* xnnext) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's copies of tile rb have landed
+    if (wave == 0) xn[lane] = xreg;
+    uint64_t valid = r1 - rb >= 64 ? ~0ULL : ((1ULL << (r1 - rb)) - 1ULL);
+    valid &= wreg;
+    __syncthreads();                     // every wave's copies landed; the other buffer is free
+    if (rb + kTileRows < r1) {
+      issue(rb + kTileRows, Xnext);
+      side(rb + kTileRows);
+    }
+    (void)xnnext;
+    f32x4_t acc[4];
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) acc[jt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < P16; ++b) {
+      // With whole groups of 8 pieces, piece 4b + g of row j sits at
+      // 8 * (b / 2) + ((4 * (b % 2) + g) ^ (j % 8)): two per-lane bases and
+      // constant offsets, so the reads need no per-b address registers.
+      const uint32_t slot = PP % 8 == 0 ? (b % 2 ? slot1 : slot0) + 8u * (b / 2) : swz<PP>(4u * b + g, j);
+#pragma unroll
+      for (int jt = 0; jt < 4; ++jt) {
+        const float4 av = X[(16u * jt + j) * PP + slot];
+        acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bq[b].x, acc[jt], 0, 0, 0);
+        acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bq[b].y, acc[jt], 0, 0, 0);
+        acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bq[b].z, acc[jt], 0, 0, 0);
+        acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bq[b].w, acc[jt], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) {
+      const float4 nv = *reinterpret_cast<const float4*>(&xn[16 * jt + 4 * g]);
+      const float norms[4] = {nv.x, nv.y, nv.z, nv.w};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t i = 16u * jt + 4u * g + static_cast<uint32_t>(r);
+        const float dp = acc[jt][r];
+        const float v = l2 ? norms[r] - 2.0f * dp : -dp;
+        const uint32_t o = ((valid >> i) & 1ULL) ? ord_of(v) : kOrdInf;
+        const bool want = o < lo[kLaneKeep - 1];
+        if (ballot(want) != 0ULL) {
+          if (want) {
+            uint32_t co = o, ci = rb + i;
+#pragma unroll
+            for (int s = 0; s < static_cast<int>(kLaneKeep); ++s) {
+              const bool lt = co < lo[s];
+              const uint32_t to = lt ? lo[s] : co, ti = lt ? li[s] : ci;
+              lo[s] = lt ? co : lo[s];
+              li[s] = lt ? ci : li[s];
+              co = to;
+              ci = ti;
+            }
+          }
+        }
+      }
+    }
+  };
+  if (r0 < r1) {
+    issue(r0, xsA);
+    side(r0);
+  }
+  for (uint32_t rb = r0; rb < r1; rb += 2 * kTileRows) {
+    tile(rb, xsA, xnA, xsB, xnB);
+    if (rb + kTileRows < r1) tile(rb + kTileRows, xsB, xnB, xsA, xnA);
   }
   if (q < a.nq) {
     uint64_t* dst = a.cand + (static_cast<size_t>(q) * a.nchunks + chunk) * kChunkList + kLaneKeep * g;
@@ -487,28 +638,32 @@ __global__ void zero_word(uint32_t* p) {
 }
 
 template <int P16>
-void launch_mfma_t(const MfmaArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL((brute_mfma<P16>), dim3(a.qblocks * a.nchunks), dim3(512), 0, st, a);
+void launch_mfma_t(const MfmaArgs& a, int variant, hipStream_t st) {
+  if (variant == 1) {
+    hipLaunchKernelGGL((brute_mfma_dma<P16>), dim3(a.qblocks * a.nchunks), dim3(512), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((brute_mfma<P16>), dim3(a.qblocks * a.nchunks), dim3(512), 0, st, a);
+  }
 }
 
-void launch_mfma(uint32_t P, const MfmaArgs& a, hipStream_t st) {
+void launch_mfma(uint32_t P, const MfmaArgs& a, int variant, hipStream_t st) {
   switch (P / 16) {
-    case 1: launch_mfma_t<1>(a, st); break;
-    case 2: launch_mfma_t<2>(a, st); break;
-    case 3: launch_mfma_t<3>(a, st); break;
-    case 4: launch_mfma_t<4>(a, st); break;
-    case 5: launch_mfma_t<5>(a, st); break;
-    case 6: launch_mfma_t<6>(a, st); break;
-    case 7: launch_mfma_t<7>(a, st); break;
-    case 8: launch_mfma_t<8>(a, st); break;
-    case 9: launch_mfma_t<9>(a, st); break;
-    case 10: launch_mfma_t<10>(a, st); break;
-    case 11: launch_mfma_t<11>(a, st); break;
-    case 12: launch_mfma_t<12>(a, st); break;
-    case 13: launch_mfma_t<13>(a, st); break;
-    case 14: launch_mfma_t<14>(a, st); break;
-    case 15: launch_mfma_t<15>(a, st); break;
-    case 16: launch_mfma_t<16>(a, st); break;
+    case 1: launch_mfma_t<1>(a, variant, st); break;
+    case 2: launch_mfma_t<2>(a, variant, st); break;
+    case 3: launch_mfma_t<3>(a, variant, st); break;
+    case 4: launch_mfma_t<4>(a, variant, st); break;
+    case 5: launch_mfma_t<5>(a, variant, st); break;
+    case 6: launch_mfma_t<6>(a, variant, st); break;
+    case 7: launch_mfma_t<7>(a, variant, st); break;
+    case 8: launch_mfma_t<8>(a, variant, st); break;
+    case 9: launch_mfma_t<9>(a, variant, st); break;
+    case 10: launch_mfma_t<10>(a, variant, st); break;
+    case 11: launch_mfma_t<11>(a, variant, st); break;
+    case 12: launch_mfma_t<12>(a, variant, st); break;
+    case 13: launch_mfma_t<13>(a, variant, st); break;
+    case 14: launch_mfma_t<14>(a, variant, st); break;
+    case 15: launch_mfma_t<15>(a, variant, st); break;
+    case 16: launch_mfma_t<16>(a, variant, st); break;
     default: throw std::logic_error("brute_mfma: unsupported padded dimension");
   }
   check(hipGetLastError(), "brute mfma launch");
@@ -667,7 +822,7 @@ void brute_search(BruteSet* s, const float* qs, std::size_t nq, std::size_t k, c
     hipLaunchKernelGGL(zero_word, dim3(1), dim3(64), 0, st, redo);
     check(hipGetLastError(), "zero launch");
     if (s->kernel_timing) check(hipEventRecord(s->ev[1], st), "hipEventRecord");
-    launch_mfma(P, m, st);
+    launch_mfma(P, m, s->mfma_variant, st);
     if (s->kernel_timing) check(hipEventRecord(s->ev[2], st), "hipEventRecord");
     MergeArgs g{};
     g.rows = s->rows;
@@ -720,6 +875,9 @@ void brute_set_option(BruteSet* s, const std::string& name, int64_t value) {
     s->force_scan = value != 0;
   } else if (name == "kernel_timing") {
     s->kernel_timing = value != 0;
+  } else if (name == "mfma_variant") {
+    if (value < 0 || value > 1) throw std::invalid_argument("mfma_variant must be 0 or 1");
+    s->mfma_variant = static_cast<int>(value);
   } else if (name == "chunk_rows") {
     if (value < 0 || value > (1 << 30)) throw std::invalid_argument("chunk_rows out of range");
     s->chunk_rows = static_cast<uint32_t>(value);
@@ -732,6 +890,7 @@ int64_t brute_get_option(const BruteSet* s, const std::string& name) {
   if (name == "force_scan") return s->force_scan;
   if (name == "kernel_timing") return s->kernel_timing;
   if (name == "chunk_rows") return s->chunk_rows;
+  if (name == "mfma_variant") return s->mfma_variant;
   if (name == "last_mfma") return s->last_mfma;
   if (name == "last_redo") return s->last_redo;
   if (name == "last_chunks") return s->last_chunks;

@@ -399,7 +399,7 @@ void BruteForceIndex::set_device_option(const std::string& name, int64_t value) 
   }
   if (!found) {
     // Validate the name before keeping it for a device set created later.
-    if (name != "force_scan" && name != "kernel_timing" && name != "chunk_rows") {
+    if (name != "force_scan" && name != "kernel_timing" && name != "chunk_rows" && name != "mfma_variant") {
       throw std::invalid_argument("unknown brute-force option: " + name);
     }
     options_.emplace_back(name, value);
@@ -411,6 +411,7 @@ int64_t BruteForceIndex::device_option(const std::string& name) const {
   for (const auto& [n, v] : options_) {
     if (n == name) return v;
   }
+  if (name == "mfma_variant") return 1;
   if (name == "force_scan" || name == "kernel_timing" || name == "chunk_rows" || name.rfind("last_", 0) == 0) return 0;
   throw std::invalid_argument("unknown brute-force option: " + name);
 }

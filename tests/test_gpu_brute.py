@@ -83,12 +83,15 @@ def test_mfma_prefilter_integer_mixture_and_chunks(lib, oracle):
     bf = lib.BruteForceIndex(128, "l2")
     bf.add(X)
     scan, _ = both_paths(bf, Q, 10)
-    for rows in (0, 2048, 64, 60_032):
-        bf.set_option("chunk_rows", rows)
-        fast = bf.search_rows(Q, 10)
-        assert bf.get_option("last_mfma") == 1
-        assert rows_equal(fast, scan), (rows, bf.get_option("last_chunks"), bf.get_option("last_redo"))
+    for variant in (0, 1):  # register-staged and global_load_lds tiles
+        bf.set_option("mfma_variant", variant)
+        for rows in (0, 2048, 64, 60_032):
+            bf.set_option("chunk_rows", rows)
+            fast = bf.search_rows(Q, 10)
+            assert bf.get_option("last_mfma") == 1
+            assert rows_equal(fast, scan), (variant, rows, bf.get_option("last_chunks"), bf.get_option("last_redo"))
     bf.set_option("chunk_rows", 0)
+    bf.set_option("mfma_variant", 1)
     # Against the oracle's scan (AVX-512 order) on a few queries.
     want = oracle.brute_force(X, Q[:16], 10, 0, order="avx512")
     assert np.array_equal(scan["label"][:16].astype(np.int64), want["label"].astype(np.int64))
