@@ -86,6 +86,23 @@ def ground_truth(X: np.ndarray, Q: np.ndarray, k: int) -> tuple[np.ndarray, floa
     return out, agree / sample, info
 
 
+def measured_traffic(ef: int, args) -> dict:
+    """HBM bytes per fast-kernel launch from the committed rocprofv3 counter
+    runs (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration) on
+    this workload, when they cover this ef and shape; otherwise null."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "fast_search_mixture1M_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        rec = t["kernels"][str(ef)]
+        if args.n != 1_000_000 or args.nq != t["nq"] or args.k != 10 or args.set:
+            raise KeyError("different workload")
+        return {"traffic": rec["hbm_bytes"], "traffic_source": "profiles/fast_search_mixture1M_ef64_ef128.md",
+                "traffic_over_algorithmic": round(rec["hbm_bytes"] / rec["algorithmic_bytes"], 3)}
+    except (OSError, KeyError, ValueError):
+        return {"traffic": None}
+
+
 def recall_at(labels: np.ndarray, truth: np.ndarray, k: int) -> float:
     hits = 0
     for a, b in zip(labels[:, :k], truth[:, :k]):
@@ -151,6 +168,8 @@ def main() -> int:
                     help="device-layer option for every search (e.g. hash_log2=12)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (counter calibration) and exit")
+    ap.add_argument("--profile-brute", action="store_true",
+                    help="profiling run of the brute force only (n x 128 data, nq queries, k), one line")
     args = ap.parse_args()
     if args.calibrate_copy:
         import hnswgpu
@@ -178,6 +197,17 @@ def main() -> int:
     X = workloads.mixture_128(args.n, seed=5)
     Q = workloads.mixture_128(args.nq, seed=1005)
     log(f"data {X.shape} + {Q.shape} in {time.perf_counter() - t0:.1f}s")
+    if args.profile_brute:
+        bf = hnswgpu.BruteForceIndex(X.shape[1], "l2")
+        bf.add(X)
+        bf.set_option("kernel_timing", 1)
+        runs = []
+        for _ in range(3):
+            bf.search_rows(Q, k)
+            runs.append({"gemm_ms": bf.get_option("last_gemm_ns") / 1e6, "device_ms": bf.get_option("last_total_ns") / 1e6,
+                         "scan_requeries": bf.get_option("last_redo"), "chunks": bf.get_option("last_chunks")})
+        print(json.dumps({"profile_brute": True, "n": args.n, "nq": args.nq, "k": k, "runs": runs}), flush=True)
+        return 0
 
     index_path = args.index or os.path.join(tempfile.gettempdir(), f"hnsw_bench_{args.n}.idx")
     if args.index and os.path.exists(index_path):
@@ -319,7 +349,7 @@ def main() -> int:
                                f"{args.nq} queries, M=16, ef_construction=200, k={k}, ef sweep {efs}",
                    "n": args.n, "nq": args.nq, "k": k, "efs": efs, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": op["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(op["hbm_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(op["hbm_gbs"] / HBM_PEAK_GBS, 4), **measured_traffic(op["ef"], args),
                      "kernel": "fast_search", "at_ef": op["ef"], "kernel_ms": op["kernel_ms"],
                      "bytes_per_query": op["bytes_per_query"]},
         "cpu_baseline": cpu,
