@@ -1,15 +1,6 @@
 // The device layer of the C ABI (include/hnswgpu.h, hnswgpu_*): uploads a
-// loaded index into HBM once and serves search_batch from it.
-//
-// HBM layout of one index (all arrays separate allocations, 256-byte aligned):
-//   vecs       count x padded fp32       rows of padded*4 bytes (a multiple of 64)
-//   l0         count x (1 + 2M) u32      [count][ids...] per node
-//   upper      upper_used u32            per node of level L: L blocks of (1 + M)
-//   upper_off  count u32                 prefix sum of level * (1 + M)
-//   labels     count u64
-//   deleted    ceil(count / 32) u32      bit set = tombstone
-// plus per-handle scratch grown on demand (prepared queries, eligibility
-// bitmap, overflow queue, counters, general-kernel heaps).
+// loaded index into HBM once and serves search_batch from it. The HBM layout
+// is described in index_state.hpp.
 
 #include <hip/hip_runtime.h>
 
@@ -23,41 +14,18 @@
 
 #include "../../../include/hnswgpu.h"
 #include "../error.hpp"
+#include "index_state.hpp"
 #include "launch.hpp"
 
+using hnswamd::dev::check;
+using hnswamd::dev::DevBuf;
 using hnswamd::gpu::DevIndexView;
 using hnswamd::gpu::OutRow;
+using hnswamd::gpu::QueryTask;
 using hnswamd::gpu::SearchRequest;
 using hnswamd::gpu::SearchScratch;
 
 namespace {
-
-void check(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw hnswamd::DeviceError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-/// A device allocation that grows (discarding contents) when asked for more.
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  void* need(size_t n) {
-    if (n <= bytes && p != nullptr) return p;
-    if (p != nullptr) {
-      (void)hipFree(p);
-      p = nullptr;
-      bytes = 0;
-    }
-    const size_t alloc = std::max<size_t>(n, 256);
-    check(hipMalloc(&p, alloc), "hipMalloc");
-    bytes = alloc;
-    return p;
-  }
-  void release() {
-    if (p != nullptr) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-};
 
 uint32_t next_log2(uint64_t v) {
   uint32_t l = 0;
@@ -66,54 +34,6 @@ uint32_t next_log2(uint64_t v) {
 }
 
 }  // namespace
-
-struct hnswgpu_index {
-  std::mutex mu;
-  DevIndexView view{};
-  uint64_t dim = 0;
-  uint64_t ef_search = 10;
-  uint64_t num_deleted = 0;
-  uint64_t upper_used = 0;
-  std::vector<uint8_t> deleted_host;  // mirror of the device bits, for counting
-
-  DevBuf vecs, l0, upper, upper_off, labels, deleted;
-  DevBuf vlow, vscale, vnorm;  // reduced-precision traversal copy (storage option)
-  // scratch
-  DevBuf counters, overflow, qprep, elig, spill_bits, spill_heaps, host_qs, host_out, host_counts, host_filter;
-  uint32_t spill_waves = 64;
-  // HBM second level of the visited set: one table per resident wavefront,
-  // kept empty between queries by the kernel itself.
-  DevBuf visit_tables;
-  size_t visit_tables_cleared = 0;
-  uint32_t visit_log2 = 15;
-  uint32_t waves_per_cu = 16;
-  uint32_t last_counters[8] = {};
-  uint64_t spill_alloc_count = 0;
-
-  // options
-  uint32_t hash_log2 = 0;
-  int force_spill = 0;
-  int collect_stats = 0;
-  int kernel_timing = 0;
-  uint32_t last_overflow = 0;
-  uint64_t last_nq = 0;
-  hipStream_t stream = nullptr;
-  DevBuf stats;
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  bool timing_pending = false;
-
-  ~hnswgpu_index() {
-    for (DevBuf* b : {&vecs, &l0, &upper, &upper_off, &labels, &deleted, &counters, &overflow, &qprep, &elig,
-                      &spill_bits, &spill_heaps, &host_qs, &host_out, &host_counts, &host_filter,
-                      &visit_tables, &vlow, &vscale, &vnorm}) {
-      b->release();
-    }
-    stats.release();
-    if (ev_begin != nullptr) (void)hipEventDestroy(ev_begin);
-    if (ev_end != nullptr) (void)hipEventDestroy(ev_end);
-    if (stream != nullptr) (void)hipStreamDestroy(stream);
-  }
-};
 
 namespace {
 
@@ -219,36 +139,15 @@ uint32_t resident_waves(size_t lds_per_wave, uint32_t wpb, uint32_t waves_per_cu
   return 256u * std::max<uint32_t>(wpb, std::min(by_lds, waves_per_cu));
 }
 
-void ensure_spill(hnswgpu_index& h) {
-  const uint64_t n = h.view.count;
-  if (h.spill_alloc_count == n && h.spill_bits.p != nullptr) return;
-  const uint64_t words = (n + 31) / 32;
-  h.spill_bits.need(static_cast<size_t>(h.spill_waves) * words * sizeof(uint32_t));
-  h.spill_heaps.need(static_cast<size_t>(h.spill_waves) * (2 * n + 4) * sizeof(uint64_t));
-  h.spill_alloc_count = n;
-}
-
 /// Device-side search on device buffers (caller holds h.mu).
 void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
                           const uint64_t* d_fw, uint64_t fsize, int by_id, hnswgpu_neighbor* d_out,
                           uint32_t* d_counts, hipStream_t st) {
-  const uint64_t ef_eff = std::max<uint64_t>(ef == 0 ? h.ef_search : ef, k);
+  const uint64_t ef_eff = hnswamd::dev::effective_ef(h, k, ef);
   if (ef_eff > 0xFFFFFFFFull || k > 0xFFFFFFFFull || nq > 0xFFFFFFFFull) {
     throw std::invalid_argument("search_batch: size out of range");
   }
   const DevIndexView& ix = h.view;
-  auto* counters = static_cast<uint32_t*>(h.counters.need(8 * sizeof(uint32_t)));
-  SearchScratch s{};
-  s.counters = counters;
-  s.overflow = static_cast<uint32_t*>(h.overflow.need(nq * sizeof(uint32_t)));
-  s.qprep = static_cast<float*>(h.qprep.need(nq * ix.padded * sizeof(float)));
-  ensure_spill(h);
-  s.spill_words = static_cast<uint32_t*>(h.spill_bits.p);
-  s.spill_heaps = static_cast<uint64_t*>(h.spill_heaps.p);
-  s.spill_heap_stride = 2ULL * ix.count + 4;
-  s.spill_waves = h.spill_waves;
-  s.spill_bitmap_words = (ix.count + 31) / 32;
-
   SearchRequest r{};
   r.queries = d_qs;
   r.dim = static_cast<uint32_t>(h.dim);
@@ -257,31 +156,16 @@ void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint
   r.ef = static_cast<uint32_t>(ef_eff);
   r.out = reinterpret_cast<OutRow*>(d_out);
   r.counts = d_counts;
-  r.stats = h.collect_stats ? static_cast<uint32_t*>(h.stats.need(nq * 3 * sizeof(uint32_t))) : nullptr;
+  r.stats = h.collect_stats ? static_cast<uint32_t*>(h.main.stats.need(nq * 3 * sizeof(uint32_t))) : nullptr;
   h.last_nq = nq;
   r.force_spill = h.force_spill;
-  r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(r.ef);
-  const size_t per_wave = static_cast<size_t>(hnswamd::gpu::lds_words_per_wave(r.hash_log2, ix.padded)) * 4u;
-  r.waves_per_block = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / per_wave)));
-  // A persistent grid of the resident wavefronts (each owns an HBM visited table).
-  const uint32_t waves = std::min<uint32_t>(resident_waves(per_wave, r.waves_per_block, h.waves_per_cu),
-                                            static_cast<uint32_t>((nq + r.waves_per_block - 1) /
-                                                                  r.waves_per_block * r.waves_per_block));
-  r.blocks = std::max<uint32_t>(1, waves / r.waves_per_block);
-  const size_t table_words = static_cast<size_t>(1) << h.visit_log2;
-  s.visit_log2 = h.visit_log2;
-  s.visit_tables = static_cast<uint32_t*>(
-      h.visit_tables.need(static_cast<size_t>(r.blocks) * r.waves_per_block * table_words * sizeof(uint32_t)));
-  if (h.visit_tables_cleared < h.visit_tables.bytes) {
-    check(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(h.visit_tables.p), 0xFFFFFFFFu,
-                            h.visit_tables.bytes / sizeof(uint32_t), st), "clear visited tables");
-    h.visit_tables_cleared = h.visit_tables.bytes;
-  }
+  hnswamd::dev::plan_request(h, r, r.nq);
+  SearchScratch s = hnswamd::dev::bind_scratch(h, h.main, r, r.nq, h.spill_waves, st);
 
   const bool filtered = d_fw != nullptr || h.num_deleted > 0;
   r.elig = nullptr;
   if (filtered) {
-    auto* e = static_cast<uint32_t*>(h.elig.need(((ix.count + 31) / 32) * sizeof(uint32_t)));
+    auto* e = static_cast<uint32_t*>(h.main.elig.need(((ix.count + 31) / 32) * sizeof(uint32_t)));
     check(hnswamd::gpu::launch_eligibility(ix, static_cast<const uint32_t*>(h.deleted.p), d_fw, fsize, by_id, e, st),
           "eligibility kernel");
     r.elig = e;
@@ -298,32 +182,6 @@ void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint
   h.timing_pending = b != nullptr;
 }
 
-/// Build (or drop) the reduced-precision traversal copy of the rows.
-void set_storage(hnswgpu_index& h, int storage) {
-  if (storage == h.view.storage) return;
-  h.view.storage = 0;
-  h.view.vlow = nullptr;
-  h.view.vscale = nullptr;
-  h.view.vnorm = nullptr;
-  h.vlow.release();
-  h.vscale.release();
-  h.vnorm.release();
-  if (storage == 0 || h.view.count == 0) {
-    h.view.storage = storage;
-    return;
-  }
-  const size_t n = h.view.count, p = h.view.padded;
-  void* low = h.vlow.need(n * p * (storage == 1 ? 2 : 1));
-  auto* scale = static_cast<float*>(h.vscale.need(n * sizeof(float)));
-  auto* norm = static_cast<float*>(h.vnorm.need(n * sizeof(float)));
-  check(hnswamd::gpu::launch_build_storage(h.view, storage, low, scale, norm, h.stream), "build storage");
-  check(hipStreamSynchronize(h.stream), "build storage");
-  h.view.storage = storage;
-  h.view.vlow = low;
-  h.view.vscale = scale;
-  h.view.vnorm = norm;
-}
-
 template <typename F>
 int guarded(F&& f) {
   try {
@@ -335,6 +193,151 @@ int guarded(F&& f) {
 }
 
 }  // namespace
+
+namespace hnswamd {
+namespace dev {
+
+uint64_t effective_ef(const hnswgpu_index& h, uint64_t k, uint64_t ef) {
+  return std::max<uint64_t>(ef == 0 ? h.ef_search : ef, k);
+}
+
+gpu::DevIndexView view_for(const hnswgpu_index& h, int storage) {
+  gpu::DevIndexView v = h.view;
+  v.storage = 0;
+  v.vlow = nullptr;
+  v.vscale = nullptr;
+  v.vnorm = nullptr;
+  if (storage == 0 || v.count == 0) return v;
+  if (storage < 0 || storage > 2 || !h.reduced[storage].built) {
+    throw std::invalid_argument("storage " + std::to_string(storage) + " is not built on this index");
+  }
+  v.storage = storage;
+  v.vlow = h.reduced[storage].low.p;
+  v.vscale = static_cast<const float*>(h.reduced[storage].scale.p);
+  v.vnorm = static_cast<const float*>(h.reduced[storage].norm.p);
+  return v;
+}
+
+void build_reduced(hnswgpu_index& h, int storage) {
+  if (storage < 1 || storage > 2) return;
+  Reduced& red = h.reduced[storage];
+  if (red.built || h.view.count == 0) return;
+  const size_t n = h.view.count, p = h.view.padded;
+  void* low = red.low.need(n * p * (storage == 1 ? 2 : 1));
+  auto* scale = static_cast<float*>(red.scale.need(n * sizeof(float)));
+  auto* norm = static_cast<float*>(red.norm.need(n * sizeof(float)));
+  gpu::DevIndexView base = h.view;
+  check(gpu::launch_build_storage(base, storage, low, scale, norm, h.stream), "build storage");
+  check(hipStreamSynchronize(h.stream), "build storage");
+  red.built = true;
+}
+
+void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq) {
+  r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(std::min<uint32_t>(r.ef, gpu::kFastMaxEf));
+  const size_t per_wave = static_cast<size_t>(gpu::lds_words_per_wave(r.hash_log2, h.view.padded)) * 4u;
+  r.waves_per_block = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / per_wave)));
+  // A persistent grid of the resident wavefronts (each owns an HBM visited table).
+  const uint32_t waves = std::min<uint32_t>(resident_waves(per_wave, r.waves_per_block, h.waves_per_cu),
+                                            (nq + r.waves_per_block - 1) / r.waves_per_block * r.waves_per_block);
+  r.blocks = std::max<uint32_t>(1, waves / r.waves_per_block);
+}
+
+gpu::SearchScratch bind_scratch(hnswgpu_index& h, Scratch& sc, const gpu::SearchRequest& r, uint32_t nq,
+                                uint32_t spill_waves, hipStream_t st) {
+  const gpu::DevIndexView& ix = h.view;
+  gpu::SearchScratch s{};
+  s.counters = static_cast<uint32_t*>(sc.counters.need(8 * sizeof(uint32_t)));
+  s.overflow = static_cast<uint32_t*>(sc.overflow.need(static_cast<size_t>(nq) * sizeof(uint32_t)));
+  s.qprep = static_cast<float*>(sc.qprep.need(static_cast<size_t>(nq) * ix.padded * sizeof(float)));
+  const uint64_t n = ix.count;
+  if (sc.spill_alloc_count != n || sc.spill_alloc_waves != spill_waves || sc.spill_bits.p == nullptr) {
+    sc.spill_bits.need(static_cast<size_t>(spill_waves) * ((n + 31) / 32) * sizeof(uint32_t));
+    sc.spill_heaps.need(static_cast<size_t>(spill_waves) * (2 * n + 4) * sizeof(uint64_t));
+    sc.spill_alloc_count = n;
+    sc.spill_alloc_waves = spill_waves;
+  }
+  s.spill_words = static_cast<uint32_t*>(sc.spill_bits.p);
+  s.spill_heaps = static_cast<uint64_t*>(sc.spill_heaps.p);
+  s.spill_heap_stride = 2ULL * n + 4;
+  s.spill_waves = spill_waves;
+  s.spill_bitmap_words = static_cast<uint32_t>((n + 31) / 32);
+  const size_t table_words = static_cast<size_t>(1) << h.visit_log2;
+  s.visit_log2 = h.visit_log2;
+  const size_t want = static_cast<size_t>(r.blocks) * r.waves_per_block * table_words * sizeof(uint32_t);
+  if (sc.visit_tables.bytes < want) sc.visit_tables_cleared = 0;
+  s.visit_tables = static_cast<uint32_t*>(sc.visit_tables.need(want));
+  if (sc.visit_tables_cleared < sc.visit_tables.bytes) {
+    // Once per allocation; the kernels leave every table empty after each query.
+    check(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(sc.visit_tables.p), 0xFFFFFFFFu,
+                            sc.visit_tables.bytes / sizeof(uint32_t), st), "clear visited tables");
+    sc.visit_tables_cleared = sc.visit_tables.bytes;
+  }
+  return s;
+}
+
+void launch_tasks(hnswgpu_index& h, Scratch& sc, const hnswgpu_task* tasks, uint64_t n,
+                  const hnswgpu_filter* filters, uint64_t nf, int storage, gpu::QueryTask* stage,
+                  gpu::QueryTask* d_tasks, uint32_t spill_waves, hipStream_t st) {
+  if (n == 0) return;
+  if (n > 0x7FFFFFFFull) throw std::invalid_argument("search_tasks: too many tasks");
+  if (tasks == nullptr || (nf > 0 && filters == nullptr)) throw std::invalid_argument("search_tasks: null pointer");
+  const gpu::DevIndexView ix = view_for(h, storage);
+  if (ix.count == 0) throw std::invalid_argument("search_tasks: empty index on the device path");
+  const size_t words = (ix.count + 31) / 32;
+  // Eligibility bitmaps: one per filter that a task uses, and one for
+  // deletions alone when there are deletions.
+  std::vector<int> used(nf, 0);
+  for (uint64_t i = 0; i < n; ++i) {
+    const hnswgpu_task& t = tasks[i];
+    if (t.k == 0) throw std::invalid_argument("search_tasks: k must be > 0");
+    if (t.query == nullptr || t.out == nullptr) throw std::invalid_argument("search_tasks: null pointer");
+    if (t.filter < -1 || (t.filter >= 0 && static_cast<uint64_t>(t.filter) >= nf)) {
+      throw std::invalid_argument("search_tasks: filter index out of range");
+    }
+    if (t.filter >= 0) used[t.filter] = 1;
+  }
+  auto* elig = static_cast<uint32_t*>(sc.elig.need((nf + 1) * words * sizeof(uint32_t)));
+  const auto* del = static_cast<const uint32_t*>(h.deleted.p);
+  for (uint64_t f = 0; f < nf; ++f) {
+    if (!used[f]) continue;
+    check(gpu::launch_eligibility(ix, del, filters[f].words, filters[f].size, 0, elig + f * words, st),
+          "eligibility kernel");
+  }
+  const uint32_t* del_only = nullptr;
+  if (h.num_deleted > 0) {
+    check(gpu::launch_eligibility(ix, del, nullptr, 0, 0, elig + nf * words, st), "eligibility kernel");
+    del_only = elig + nf * words;
+  }
+  uint64_t max_ef = 1;
+  for (uint64_t i = 0; i < n; ++i) {
+    const hnswgpu_task& t = tasks[i];
+    const uint64_t ef = effective_ef(h, t.k, t.ef);
+    if (ef > 0xFFFFFFFFull) throw std::invalid_argument("search_tasks: ef out of range");
+    max_ef = std::max(max_ef, ef);
+    gpu::QueryTask q{};
+    q.query = t.query;
+    q.out = reinterpret_cast<gpu::OutRow*>(t.out);
+    q.count = t.count;
+    q.elig = t.filter >= 0 ? elig + static_cast<size_t>(t.filter) * words : del_only;
+    q.k = t.k;
+    q.ef = static_cast<uint32_t>(ef);
+    stage[i] = q;
+  }
+  check(hipMemcpyAsync(d_tasks, stage, n * sizeof(gpu::QueryTask), hipMemcpyHostToDevice, st), "hipMemcpy tasks");
+  gpu::SearchRequest r{};
+  r.tasks = d_tasks;
+  r.dim = static_cast<uint32_t>(h.dim);
+  r.nq = static_cast<uint32_t>(n);
+  r.ef = static_cast<uint32_t>(std::min<uint64_t>(max_ef, 0xFFFFFFFFull));
+  r.force_spill = h.force_spill;
+  plan_request(h, r, r.nq);
+  gpu::SearchScratch s = bind_scratch(h, sc, r, r.nq, spill_waves, st);
+  check(gpu::launch_prepare(ix, r, s, st), "prepare kernel");
+  check(gpu::launch_search(ix, r, s, st), "search kernel");
+}
+
+}  // namespace dev
+}  // namespace hnswamd
 
 extern "C" {
 
@@ -427,7 +430,7 @@ void device_search_host(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t
       check(hipMemcpyAsync(counts, d_counts, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "hipMemcpy counts");
     }
     uint32_t ctr[8] = {};
-    check(hipMemcpyAsync(ctr, h->counters.p, sizeof(ctr), hipMemcpyDeviceToHost, st), "hipMemcpy counters");
+    check(hipMemcpyAsync(ctr, h->main.counters.p, sizeof(ctr), hipMemcpyDeviceToHost, st), "hipMemcpy counters");
     check(hipStreamSynchronize(st), "search_batch");
     h->last_overflow = ctr[1];
     std::copy(ctr, ctr + 8, h->last_counters);
@@ -460,6 +463,24 @@ int hnswgpu_search_batch_device(hnswgpu_index* h, const float* d_qs, uint64_t nq
   });
 }
 
+int hnswgpu_search_tasks_device(hnswgpu_index* h, const hnswgpu_task* tasks, uint64_t ntasks,
+                                const hnswgpu_filter* filters, uint64_t nfilters, int32_t storage, void* stream) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("search_tasks: null handle");
+    if (ntasks == 0) return;
+    std::lock_guard<std::mutex> lk(h->mu);
+    // The pinned staging of the previous call may still be in its copy.
+    if (h->task_copied == nullptr) check(hipEventCreateWithFlags(&h->task_copied, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventSynchronize(h->task_copied), "hipEventSynchronize");
+    auto* stage = static_cast<QueryTask*>(h->task_host.need(ntasks * sizeof(QueryTask)));
+    auto* d_tasks = static_cast<QueryTask*>(h->task_dev.need(ntasks * sizeof(QueryTask)));
+    const auto st = static_cast<hipStream_t>(stream);
+    hnswamd::dev::launch_tasks(*h, h->main, tasks, ntasks, filters, nfilters, storage < 0 ? h->view.storage : storage,
+                               stage, d_tasks, h->spill_waves, st);
+    check(hipEventRecord(h->task_copied, st), "hipEventRecord");
+  });
+}
+
 int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
   return guarded([&] {
     if (h == nullptr || name == nullptr) throw std::invalid_argument("set_option: null pointer");
@@ -474,12 +495,13 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
       // 0 = traverse the fp32 rows; 1 = a bf16 copy; 2 = an fp8 e4m3 copy with
       // per-vector scales. Reduced copies re-rank the final candidates in fp32.
       if (value < 0 || value > 2) throw std::invalid_argument("storage must be 0 (fp32), 1 (bf16) or 2 (fp8)");
-      set_storage(*h, static_cast<int>(value));
+      hnswamd::dev::build_reduced(*h, static_cast<int>(value));
+      h->view = hnswamd::dev::view_for(*h, static_cast<int>(value));
     } else if (n == "visit_log2") {
       if (value < 8 || value > 24) throw std::invalid_argument("visit_log2 must be 8..24");
       h->visit_log2 = static_cast<uint32_t>(value);
-      h->visit_tables.release();
-      h->visit_tables_cleared = 0;
+      h->main.visit_tables.release();
+      h->main.visit_tables_cleared = 0;
     } else if (n == "waves_per_cu") {
       if (value < 1 || value > 32) throw std::invalid_argument("waves_per_cu must be 1..32");
       h->waves_per_cu = static_cast<uint32_t>(value);
@@ -490,9 +512,6 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
     } else if (n == "spill_waves") {
       if (value < 1 || value > 4096) throw std::invalid_argument("spill_waves must be 1..4096");
       h->spill_waves = static_cast<uint32_t>(value);
-      h->spill_bits.release();
-      h->spill_heaps.release();
-      h->spill_alloc_count = 0;
     } else {
       throw std::invalid_argument("unknown option " + n);
     }
@@ -510,10 +529,10 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
       *value = h->force_spill;
     } else if (n == "spill_waves") {
       *value = h->spill_waves;
-    } else if (n.rfind("last_", 0) == 0 && n != "last_kernel_ns" && h->counters.p != nullptr) {
+    } else if (n.rfind("last_", 0) == 0 && n != "last_kernel_ns" && h->main.counters.p != nullptr) {
       // Counters of the most recent search on this handle (waits for it).
       check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-      check(hipMemcpy(h->last_counters, h->counters.p, sizeof(h->last_counters), hipMemcpyDeviceToHost),
+      check(hipMemcpy(h->last_counters, h->main.counters.p, sizeof(h->last_counters), hipMemcpyDeviceToHost),
             "hipMemcpy counters");
       h->last_overflow = h->last_counters[1];
       if (n == "last_overflow") {
@@ -568,10 +587,10 @@ extern "C" int hnswgpu_read_stats(hnswgpu_index* h, uint32_t* out, uint64_t nq) 
   return guarded([&] {
     if (h == nullptr || out == nullptr) throw std::invalid_argument("read_stats: null pointer");
     std::lock_guard<std::mutex> lk(h->mu);
-    if (!h->collect_stats || h->stats.p == nullptr) throw std::invalid_argument("read_stats: collect_stats is off");
+    if (!h->collect_stats || h->main.stats.p == nullptr) throw std::invalid_argument("read_stats: collect_stats is off");
     if (nq > h->last_nq) throw std::invalid_argument("read_stats: more queries than the last search had");
     check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    check(hipMemcpy(out, h->stats.p, nq * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy stats");
+    check(hipMemcpy(out, h->main.stats.p, nq * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy stats");
   });
 }
 
@@ -692,14 +711,14 @@ extern "C" int hnswgpu_get_row(hnswgpu_index* h, uint32_t id, int storage, void*
       check(hipMemcpy(out, h->view.vecs + id * p, p * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy row");
       return;
     }
-    if (storage != h->view.storage) throw std::invalid_argument("get_row: that storage is not built");
+    const DevIndexView v = hnswamd::dev::view_for(*h, storage);
     const size_t width = storage == 1 ? 2 : 1;
-    check(hipMemcpy(out, static_cast<const uint8_t*>(h->view.vlow) + id * p * width, p * width,
+    check(hipMemcpy(out, static_cast<const uint8_t*>(v.vlow) + id * p * width, p * width,
                     hipMemcpyDeviceToHost), "hipMemcpy row");
     if (scale_norm != nullptr) {
       float sn[2] = {1.0f, 0.0f};
-      if (storage == 2) check(hipMemcpy(&sn[0], h->view.vscale + id, sizeof(float), hipMemcpyDeviceToHost), "scale");
-      check(hipMemcpy(&sn[1], h->view.vnorm + id, sizeof(float), hipMemcpyDeviceToHost), "norm");
+      if (storage == 2) check(hipMemcpy(&sn[0], v.vscale + id, sizeof(float), hipMemcpyDeviceToHost), "scale");
+      check(hipMemcpy(&sn[1], v.vnorm + id, sizeof(float), hipMemcpyDeviceToHost), "norm");
       scale_norm[0] = sn[0];
       scale_norm[1] = sn[1];
     }

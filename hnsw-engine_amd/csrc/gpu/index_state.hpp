@@ -1,0 +1,167 @@
+#pragma once
+
+// Host-side state of a resident index (the hnswgpu_index handle) and the
+// per-launch scratch, shared by the C-ABI layer (device_index.hip) and the
+// serving loop (server.hip).
+//
+// HBM layout of one index (all arrays separate allocations, 256-byte aligned):
+//   vecs       count x padded fp32       rows of padded*4 bytes (a multiple of 64)
+//   l0         count x (1 + 2M) u32      [count][ids...] per node
+//   upper      upper_used u32            per node of level L: L blocks of (1 + M)
+//   upper_off  count u32                 prefix sum of level * (1 + M)
+//   labels     count u64
+//   deleted    ceil(count / 32) u32      bit set = tombstone
+//   reduced    per built storage: count x padded bf16 (2 B) or e4m3 (1 B), scales, |x^|^2
+// Scratch (one set per launch in flight): counters, overflow queue, prepared
+// queries, eligibility bitmaps, HBM visited tables (one per resident
+// wavefront), general-kernel bitmaps and heaps.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../../include/hnswgpu.h"
+#include "../error.hpp"
+#include "launch.hpp"
+
+namespace hnswamd {
+namespace dev {
+
+inline void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+/// A device allocation that grows (discarding contents) when asked for more.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* need(size_t n) {
+    if (n <= bytes && p != nullptr) return p;
+    release();
+    const size_t alloc = std::max<size_t>(n, 256);
+    check(hipMalloc(&p, alloc), "hipMalloc");
+    bytes = alloc;
+    return p;
+  }
+  void release() {
+    if (p != nullptr) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+};
+
+/// Pinned host memory (hipHostMalloc) that grows when asked for more.
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* need(size_t n) {
+    if (n <= bytes && p != nullptr) return p;
+    release();
+    const size_t alloc = std::max<size_t>(n, 256);
+    check(hipHostMalloc(&p, alloc, hipHostMallocDefault), "hipHostMalloc");
+    bytes = alloc;
+    return p;
+  }
+  void release() {
+    if (p != nullptr) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+};
+
+/// Scratch of one launch in flight. Launches that may overlap on the device
+/// need separate Scratch objects (each holds the visited tables of the
+/// wavefronts of its kernel).
+struct Scratch {
+  DevBuf counters, overflow, qprep, elig, spill_bits, spill_heaps, visit_tables, stats;
+  size_t visit_tables_cleared = 0;
+  uint64_t spill_alloc_count = 0;
+  uint32_t spill_alloc_waves = 0;
+};
+
+/// A reduced-precision traversal copy of the rows.
+struct Reduced {
+  DevBuf low, scale, norm;
+  bool built = false;
+};
+
+}  // namespace dev
+}  // namespace hnswamd
+
+struct hnswgpu_index {
+  std::mutex mu;
+  hnswamd::gpu::DevIndexView view{};  // view.storage: the copy search_batch traverses
+  uint64_t dim = 0;
+  uint64_t ef_search = 10;
+  uint64_t num_deleted = 0;
+  uint64_t upper_used = 0;
+  std::vector<uint8_t> deleted_host;  // mirror of the device bits, for counting
+
+  hnswamd::dev::DevBuf vecs, l0, upper, upper_off, labels, deleted;
+  hnswamd::dev::Reduced reduced[3];  // [1] bf16, [2] fp8 e4m3 + scale
+  hnswamd::dev::Scratch main;        // scratch of the synchronous entry points
+  hnswamd::dev::DevBuf host_qs, host_out, host_counts, host_filter, task_dev;
+  hnswamd::dev::PinnedBuf task_host;
+  hipEvent_t task_copied = nullptr;  // the pinned task staging may be reused after this
+
+  // options
+  uint32_t spill_waves = 64;
+  uint32_t visit_log2 = 15;
+  uint32_t waves_per_cu = 16;
+  uint32_t hash_log2 = 0;
+  int force_spill = 0;
+  int collect_stats = 0;
+  int kernel_timing = 0;
+
+  uint32_t last_counters[8] = {};
+  uint32_t last_overflow = 0;
+  uint64_t last_nq = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  bool timing_pending = false;
+
+  ~hnswgpu_index() {
+    if (task_copied != nullptr) (void)hipEventDestroy(task_copied);
+    if (ev_begin != nullptr) (void)hipEventDestroy(ev_begin);
+    if (ev_end != nullptr) (void)hipEventDestroy(ev_end);
+    if (stream != nullptr) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace hnswamd {
+namespace dev {
+
+/// The index as the kernels see it, traversing `storage` (0 fp32, 1 bf16, 2
+/// fp8); the reduced copy must be built.
+gpu::DevIndexView view_for(const hnswgpu_index& h, int storage);
+/// Build the reduced copy for `storage` if it is not built yet (synchronous).
+void build_reduced(hnswgpu_index& h, int storage);
+/// Launch shape for r.ef and nq queries: hash size, wavefronts per block and a
+/// persistent grid of the resident wavefronts.
+void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq);
+/// Scratch for a launch planned by plan_request with room for nq queries.
+gpu::SearchScratch bind_scratch(hnswgpu_index& h, Scratch& sc, const gpu::SearchRequest& r, uint32_t nq,
+                                uint32_t spill_waves, hipStream_t st);
+/// Effective ef of a request: max(ef or ef_search, k).
+uint64_t effective_ef(const hnswgpu_index& h, uint64_t k, uint64_t ef);
+/// Mixed launch of n tasks on `st`: eligibility bitmaps for the filters the
+/// tasks use, the task array staged in pinned `stage` and copied to
+/// `d_tasks`, then prepare + fast + general kernels with scratch `sc`.
+void launch_tasks(hnswgpu_index& h, Scratch& sc, const hnswgpu_task* tasks, uint64_t n,
+                  const hnswgpu_filter* filters, uint64_t nf, int storage, gpu::QueryTask* stage,
+                  gpu::QueryTask* d_tasks, uint32_t spill_waves, hipStream_t st);
+
+}  // namespace dev
+}  // namespace hnswamd

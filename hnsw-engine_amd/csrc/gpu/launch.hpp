@@ -57,7 +57,21 @@ struct SearchScratch {
   uint32_t spill_bitmap_words;
 };
 
+/// One query of a mixed launch (device work queue entry): its own k, ef,
+/// eligibility bitmap and output rows, all device pointers.
+struct QueryTask {
+  const float* query;        // dim floats
+  OutRow* out;               // k rows
+  uint32_t* count;           // result count, or null
+  const uint32_t* elig;      // id bitmap (deletions and filter combined), or null = unfiltered
+  uint32_t k;
+  uint32_t ef;               // clamped: max(ef or ef_search, k)
+};
+
 struct SearchRequest {
+  const QueryTask* tasks;    // when set: per-query parameters; k / ef / elig / out / counts / queries
+                             // below are unused and ef holds the largest task ef
+  const uint32_t* nq_dev;    // when set: the number of queries is read here (nq is then a capacity)
   const float* queries;      // nq x dim (device)
   uint32_t dim;
   uint32_t nq;

@@ -334,17 +334,60 @@ __device__ __forceinline__ void write_pad(OutRow* row, uint32_t from, uint32_t k
   }
 }
 
+/// The parameters of query qi: from its task in a mixed launch, else from the
+/// request. Wave-uniform.
+struct QueryArgs {
+  const float* query;
+  OutRow* out;
+  uint32_t* count;
+  const uint32_t* elig;
+  uint32_t* stats;
+  uint32_t k;
+  uint32_t ef;
+};
+
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  return reinterpret_cast<T*>(uniform64(reinterpret_cast<uint64_t>(p)));
+}
+
+__device__ __forceinline__ QueryArgs query_args(const SearchRequest& r, uint32_t qi) {
+  QueryArgs a;
+  if (r.tasks != nullptr) {
+    const QueryTask& t = r.tasks[qi];
+    a.query = uniform_ptr(t.query);
+    a.out = uniform_ptr(t.out);
+    a.count = uniform_ptr(t.count);
+    a.elig = uniform_ptr(t.elig);
+    a.k = uniform(t.k);
+    a.ef = uniform(t.ef);
+  } else {
+    a.query = r.queries + static_cast<size_t>(qi) * r.dim;
+    a.out = r.out + static_cast<size_t>(qi) * r.k;
+    a.count = r.counts != nullptr ? r.counts + qi : nullptr;
+    a.elig = r.elig;
+    a.k = r.k;
+    a.ef = r.ef;
+  }
+  a.stats = r.stats != nullptr ? r.stats + 3 * static_cast<size_t>(qi) : nullptr;
+  return a;
+}
+
+__device__ __forceinline__ uint32_t launch_nq(const SearchRequest& r) {
+  return r.nq_dev != nullptr ? uniform(*r.nq_dev) : r.nq;
+}
+
 /// Why the fast kernel handed a query to the general kernel.
 enum class Handoff : uint32_t { kNone = 0, kCandidates = 4, kVisited = 5 };
 
 /// Results of a finished query: R's first min(k, |R|) entries with labels,
 /// padding, result count and work counts.
 template <int ER>
-__device__ void write_results(const DevIndexView& ix, const SearchRequest& r, const WaveSorted<ER>& R, uint32_t sR,
-                              const Work& work, uint32_t qi) {
+__device__ void write_results(const DevIndexView& ix, const QueryArgs& a, const WaveSorted<ER>& R, uint32_t sR,
+                              const Work& work) {
   const int lane = lane_id();
-  const uint32_t kout = sR < r.k ? sR : r.k;
-  OutRow* row = r.out + static_cast<size_t>(qi) * r.k;
+  const uint32_t kout = sR < a.k ? sR : a.k;
+  OutRow* row = a.out;
 #pragma unroll
   for (int e = 0; e < ER; ++e) {
     const uint32_t idx = static_cast<uint32_t>(e * 64 + lane);
@@ -353,12 +396,12 @@ __device__ void write_results(const DevIndexView& ix, const SearchRequest& r, co
       row[idx] = OutRow{ix.labels[key_id(key)], float_of(key_ord(key)), 0u};
     }
   }
-  write_pad(row, kout, r.k);
-  if (r.counts != nullptr && lane == 0) r.counts[qi] = kout;
-  if (r.stats != nullptr && lane == 0) {
-    r.stats[3 * static_cast<size_t>(qi)] = work.dist;
-    r.stats[3 * static_cast<size_t>(qi) + 1] = work.exp0;
-    r.stats[3 * static_cast<size_t>(qi) + 2] = work.expu;
+  write_pad(row, kout, a.k);
+  if (a.count != nullptr && lane == 0) *a.count = kout;
+  if (a.stats != nullptr && lane == 0) {
+    a.stats[0] = work.dist;
+    a.stats[1] = work.exp0;
+    a.stats[2] = work.expu;
   }
 }
 
@@ -384,10 +427,10 @@ __device__ __forceinline__ uint32_t first_unexpanded(const uint32_t (&fl)[ER], u
 /// tie" is the exact Alg. 2 order, and the loop ends exactly when Alg. 2 would
 /// stop (the smallest remaining candidate is further than the worst result).
 template <int ER, int PC, int ST>
-__device__ Handoff plain_query(const DevIndexView& ix, const SearchRequest& r, const SearchScratch& s,
+__device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s,
                                const WaveLds& w, Visited& vis, uint32_t qi) {
   const int lane = lane_id();
-  const uint32_t ef = r.ef;
+  const uint32_t ef = a.ef;
 
   for (uint32_t i = static_cast<uint32_t>(lane); i < (1u << vis.lds_log2); i += 64) w.hash[i] = kEmptySlot;
   vis.in_lds = 0;
@@ -481,7 +524,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const SearchRequest& r, c
   vis.finish();
   if (result != Handoff::kNone) return result;
   if constexpr (ST != kStoreF32) rerank_exact<ER, PC>(ix, w, R, sR);
-  write_results<ER>(ix, r, R, sR, work, qi);
+  write_results<ER>(ix, a, R, sR, work);
   return Handoff::kNone;
 }
 
@@ -491,11 +534,11 @@ __device__ Handoff plain_query(const DevIndexView& ix, const SearchRequest& r, c
 /// long as that entry is never the smallest live candidate, otherwise the
 /// query goes to the general kernel.
 template <int ER, int EC, int PC, int ST>
-__device__ Handoff filtered_query(const DevIndexView& ix, const SearchRequest& r, const SearchScratch& s,
+__device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s,
                                   const WaveLds& w, Visited& vis, uint32_t qi) {
   const int lane = lane_id();
   const uint32_t ccap = 64u * EC;
-  const uint32_t ef = r.ef;
+  const uint32_t ef = a.ef;
 
   for (uint32_t i = static_cast<uint32_t>(lane); i < (1u << vis.lds_log2); i += 64) w.hash[i] = kEmptySlot;
   vis.in_lds = 0;
@@ -515,7 +558,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const SearchRequest& r
   wave_sync();
   C.insert(ep, ccap);
   sC = 1;
-  if (elig_bit(r.elig, key_id(ep))) {
+  if (elig_bit(a.elig, key_id(ep))) {
     R.insert(ep, ef);
     sR = 1;
     tau = key_ord(ep);
@@ -543,7 +586,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const SearchRequest& r
     const uint64_t peek = C.at(0);
     if (peek != kKeyInf) next = load_list0(ix, key_id(peek));
 
-    const uint32_t U = fresh_from_list(ix, w, cur, r.elig, [&](uint32_t id) { return vis.visit(id); });
+    const uint32_t U = fresh_from_list(ix, w, cur, a.elig, [&](uint32_t id) { return vis.visit(id); });
     vis.count(U);
     work.dist += U;
     ++work.exp0;
@@ -576,11 +619,34 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const SearchRequest& r
   vis.finish();
   if (result != Handoff::kNone) return result;
   if constexpr (ST != kStoreF32) rerank_exact<ER, PC>(ix, w, R, sR);
-  write_results<ER>(ix, r, R, sR, work, qi);
+  write_results<ER>(ix, a, R, sR, work);
   return Handoff::kNone;
 }
 
-template <int ER, int EC, int PC, bool kFiltered, int ST>
+/// Filter modes of the fast kernel: every query unfiltered, every query
+/// filtered, or per query (mixed launches).
+enum FilterMode : int { kPlain = 0, kFiltered = 1, kPerQuery = 2 };
+
+/// One query. In a mixed launch (kPerQuery) the kernel is built for the
+/// largest capacity class of the batch and a query with a smaller ef runs the
+/// code of its own class.
+template <int ER, int EC, int PC, int FM, int ST>
+__device__ Handoff run_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s, const WaveLds& w,
+                             Visited& vis, uint32_t qi) {
+  if constexpr (FM == kPerQuery && ER > 1) {
+    if (a.ef <= 32u * ER) return run_query<ER / 2, EC / 2, PC, FM, ST>(ix, a, s, w, vis, qi);
+  }
+  if constexpr (FM == kPlain) {
+    return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
+  } else if constexpr (FM == kFiltered) {
+    return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi);
+  } else {
+    if (a.elig == nullptr) return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
+    return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi);
+  }
+}
+
+template <int ER, int EC, int PC, int FM, int ST>
 __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView ix, SearchRequest r,
                                                                    SearchScratch s) {
   extern __shared__ uint32_t lds[];
@@ -592,18 +658,17 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView 
   Visited vis{w.hash, r.hash_log2,
               s.visit_tables != nullptr ? s.visit_tables + (wave_slot << s.visit_log2) : nullptr,
               s.visit_log2, 0u, 0u, false};
+  const uint32_t nq = launch_nq(r);
   for (;;) {
     uint32_t qi = 0;
     if (lane == 0) qi = atomicAdd(&s.counters[0], 1u);
     qi = uniform(qi);
-    if (qi >= r.nq) break;
+    if (qi >= nq) break;
     Handoff h = Handoff::kCandidates;
     if (!r.force_spill) {
-      if constexpr (kFiltered) {
-        h = filtered_query<ER, EC, PC, ST>(ix, r, s, w, vis, qi);
-      } else {
-        h = plain_query<ER, PC, ST>(ix, r, s, w, vis, qi);
-      }
+      const QueryArgs a = query_args(r, qi);
+      // A task whose ef exceeds the register capacity goes to the general kernel.
+      if (a.ef <= 64u * ER) h = run_query<ER, EC, PC, FM, ST>(ix, a, s, w, vis, qi);
     }
     if (h != Handoff::kNone && lane == 0) {
       const uint32_t slot = atomicAdd(&s.counters[1], 1u);
@@ -668,7 +733,6 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) general_search(DevIndexVi
   uint32_t* bits = s.spill_words + static_cast<size_t>(wave_slot) * s.spill_bitmap_words;
   uint64_t* Cheap = s.spill_heaps + static_cast<size_t>(wave_slot) * s.spill_heap_stride;
   uint64_t* Rheap = Cheap + (ix.count + 1u);
-  const bool filtered = r.elig != nullptr;
   const uint32_t n_over = uniform(*reinterpret_cast<volatile uint32_t*>(&s.counters[1]));
 
   for (;;) {
@@ -677,6 +741,8 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) general_search(DevIndexVi
     t = uniform(t);
     if (t >= n_over) break;
     const uint32_t qi = s.overflow[t];
+    const QueryArgs a = query_args(r, qi);
+    const bool filtered = a.elig != nullptr;
 
     for (uint32_t i = static_cast<uint32_t>(lane); i < s.spill_bitmap_words; i += 64) bits[i] = 0u;
     __threadfence();
@@ -686,14 +752,14 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) general_search(DevIndexVi
     if (lane == 0) {
       atomicOr(&bits[key_id(ep) >> 5], 1u << (key_id(ep) & 31));
       heap_push(Cheap, nC, ep, false);
-      if (!filtered || elig_bit(r.elig, key_id(ep))) heap_push(Rheap, nR, ep, true);
+      if (!filtered || elig_bit(a.elig, key_id(ep))) heap_push(Rheap, nR, ep, true);
     }
     for (;;) {
       uint64_t c = kKeyInf;
       if (lane == 0) {
         if (nC > 0) {
           c = Cheap[0];
-          if (nR >= r.ef && key_ord(c) > key_ord(Rheap[0])) {
+          if (nR >= a.ef && key_ord(c) > key_ord(Rheap[0])) {
             c = kKeyInf;
           } else {
             heap_pop(Cheap, nC, false);
@@ -702,7 +768,7 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) general_search(DevIndexVi
       }
       c = uniform64(c);
       if (c == kKeyInf) break;
-      const uint32_t U = fresh_neighbours(ix, w, key_id(c), r.elig, [&](uint32_t id) {
+      const uint32_t U = fresh_neighbours(ix, w, key_id(c), a.elig, [&](uint32_t id) {
         const uint32_t old = atomicOr(&bits[id >> 5], 1u << (id & 31));
         return (old & (1u << (id & 31))) == 0u;
       });
@@ -713,12 +779,12 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) general_search(DevIndexVi
       if (lane == 0) {
         for (uint32_t u = 0; u < U; ++u) {
           const uint32_t od = w.nd[u];
-          if (nR < r.ef || od < key_ord(Rheap[0])) {
+          if (nR < a.ef || od < key_ord(Rheap[0])) {
             const uint64_t key = key_of(od, w.nid[u]);
             heap_push(Cheap, nC, key, false);
             if (!filtered || w.nel[u] != 0u) {
               heap_push(Rheap, nR, key, true);
-              if (nR > r.ef) heap_pop(Rheap, nR, true);
+              if (nR > a.ef) heap_pop(Rheap, nR, true);
             }
           }
         }
@@ -733,18 +799,18 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) general_search(DevIndexVi
     }
     __threadfence();
     nR = uniform(nR);
-    const uint32_t kout = nR < r.k ? nR : r.k;
-    OutRow* row = r.out + static_cast<size_t>(qi) * r.k;
+    const uint32_t kout = nR < a.k ? nR : a.k;
+    OutRow* row = a.out;
     for (uint32_t j = static_cast<uint32_t>(lane); j < kout; j += 64) {
       const uint64_t key = Cheap[j];
       row[j] = OutRow{ix.labels[key_id(key)], float_of(key_ord(key)), 0u};
     }
-    write_pad(row, kout, r.k);
-    if (r.counts != nullptr && lane == 0) r.counts[qi] = kout;
-    if (r.stats != nullptr && lane == 0) {
-      r.stats[3 * static_cast<size_t>(qi)] = work.dist;
-      r.stats[3 * static_cast<size_t>(qi) + 1] = work.exp0;
-      r.stats[3 * static_cast<size_t>(qi) + 2] = work.expu;
+    write_pad(row, kout, a.k);
+    if (a.count != nullptr && lane == 0) *a.count = kout;
+    if (a.stats != nullptr && lane == 0) {
+      a.stats[0] = work.dist;
+      a.stats[1] = work.exp0;
+      a.stats[2] = work.expu;
     }
     __threadfence();
   }
@@ -762,8 +828,8 @@ __global__ void reset_counters(uint32_t* c) {
 __global__ void __launch_bounds__(256) prepare_queries(DevIndexView ix, SearchRequest r, SearchScratch s) {
   const int lane = lane_id();
   const uint32_t qi = blockIdx.x * 4u + (threadIdx.x >> 6);
-  if (qi >= r.nq) return;
-  const float* src = r.queries + static_cast<size_t>(qi) * r.dim;
+  if (qi >= launch_nq(r)) return;
+  const float* src = query_args(r, qi).query;
   float* dst = s.qprep + static_cast<size_t>(qi) * ix.padded;
   float inv = 1.0f;
   if (ix.metric == 2) {
@@ -804,33 +870,34 @@ __global__ void build_eligibility(uint32_t count, const uint64_t* labels, const 
   elig[word] = out;
 }
 
-template <int ER, int EC, int PC, bool kFiltered, int ST>
+template <int ER, int EC, int PC, int FM, int ST>
 hipError_t run_fast(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
   const size_t lds = static_cast<size_t>(r.waves_per_block) * lds_words_per_wave(r.hash_log2, ix.padded) * 4u;
   static bool configured = false;
   if (!configured) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fast_search<ER, EC, PC, kFiltered, ST>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fast_search<ER, EC, PC, FM, ST>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
     configured = true;
   }
-  hipLaunchKernelGGL((fast_search<ER, EC, PC, kFiltered, ST>), dim3(r.blocks), dim3(r.waves_per_block * 64), lds,
+  hipLaunchKernelGGL((fast_search<ER, EC, PC, FM, ST>), dim3(r.blocks), dim3(r.waves_per_block * 64), lds,
                      st, ix, r, s);
   return hipGetLastError();
 }
 
-template <int PC, bool kFiltered, int ST>
+template <int PC, int FM, int ST>
 hipError_t pick_capacity(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
-  if (r.ef <= 64) return run_fast<1, 2, PC, kFiltered, ST>(ix, r, s, st);
-  if (r.ef <= 128) return run_fast<2, 4, PC, kFiltered, ST>(ix, r, s, st);
-  if (r.ef <= 256) return run_fast<4, 8, PC, kFiltered, ST>(ix, r, s, st);
-  return run_fast<8, 16, PC, kFiltered, ST>(ix, r, s, st);
+  if (r.ef <= 64) return run_fast<1, 2, PC, FM, ST>(ix, r, s, st);
+  if (r.ef <= 128) return run_fast<2, 4, PC, FM, ST>(ix, r, s, st);
+  if (r.ef <= 256) return run_fast<4, 8, PC, FM, ST>(ix, r, s, st);
+  return run_fast<8, 16, PC, FM, ST>(ix, r, s, st);
 }
 
 template <int PC, int ST>
 hipError_t pick_filter(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
-  if (r.elig != nullptr) return pick_capacity<PC, true, ST>(ix, r, s, st);
-  return pick_capacity<PC, false, ST>(ix, r, s, st);
+  if (r.tasks != nullptr) return pick_capacity<PC, kPerQuery, ST>(ix, r, s, st);
+  if (r.elig != nullptr) return pick_capacity<PC, kFiltered, ST>(ix, r, s, st);
+  return pick_capacity<PC, kPlain, ST>(ix, r, s, st);
 }
 
 template <int PC>
@@ -893,7 +960,7 @@ hipError_t launch_build_storage(const DevIndexView& ix, int storage, void* vlow,
 }
 
 hipError_t launch_prepare(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
-  if (r.nq == 0) return hipSuccess;
+  if (r.nq == 0) return hipSuccess;  // with nq_dev, r.nq is the capacity the grid covers
   hipLaunchKernelGGL(prepare_queries, dim3((r.nq + 3) / 4), dim3(256), 0, st, ix, r, s);
   return hipGetLastError();
 }
@@ -912,9 +979,12 @@ hipError_t launch_search(const DevIndexView& ix, const SearchRequest& r, SearchS
   hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, s.counters);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const bool fast = !r.force_spill && r.ef <= kFastMaxEf;
+  // A mixed launch sizes the kernel for min(largest ef, kFastMaxEf); larger
+  // tasks are handed to the general kernel one by one.
+  const bool fast = !r.force_spill && (r.tasks != nullptr || r.ef <= kFastMaxEf);
   SearchRequest rr = r;
   rr.force_spill = fast ? 0 : 1;
+  if (rr.tasks != nullptr && rr.ef > kFastMaxEf) rr.ef = kFastMaxEf;
   if (fast_begin != nullptr && fast_end != nullptr) {
     e = hipEventRecord(fast_begin, st);
     if (e != hipSuccess) return e;

@@ -76,6 +76,7 @@ def _load():
         "hnsw_brute_force_free": ([p], None),
         "hnsw_brute_force_add": ([p, p, p, u64], C.c_int),
         "hnsw_brute_force_search_batch": ([p, p, u64, u64, p, u64, p, u64], C.c_int),
+        "hnswgpu_search_tasks_device": ([p, p, u64, p, u64, i32, p], C.c_int),
         "hnsw_brute_force_set_option": ([p, C.c_char_p, C.c_int64], C.c_int),
         "hnsw_brute_force_get_option": ([p, C.c_char_p, p], C.c_int),
     }
@@ -215,6 +216,17 @@ def device_info():
     return {"arch": name.value.decode(), **dict(zip(keys, info.tolist()))}
 
 
+class Task(C.Structure):
+    """hnswgpu_task: one query of a mixed launch (device pointers)."""
+    _fields_ = [("query", C.c_void_p), ("out", C.c_void_p), ("count", C.c_void_p), ("k", C.c_uint32),
+                ("ef", C.c_uint32), ("filter", C.c_int32), ("reserved", C.c_int32)]
+
+
+class FilterRef(C.Structure):
+    """hnswgpu_filter: BitsetFilter words on the device and their label count."""
+    _fields_ = [("words", C.c_void_p), ("size", C.c_uint64)]
+
+
 class DeviceIndex:
     """The device layer of an index (hnswgpu_*). Owned by its Index."""
 
@@ -251,6 +263,36 @@ class DeviceIndex:
         out = np.empty((nq, 3), dtype=np.uint32)
         _check(_lib.hnswgpu_read_stats(self._h, _ptr(out), nq))
         return out
+
+    def search_mixed(self, queries, ks, efs, filter_ids=None, filters=(), storage: int = -1):
+        """One launch for queries with their own k, ef and filter (a filter id
+        indexes `filters`, -1 = none). Returns a list of (rows, count)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        n, dim = q.shape
+        ks = [int(k) for k in ks]
+        efs = [int(e) for e in efs]
+        fids = [-1] * n if filter_ids is None else [int(f) for f in filter_ids]
+        d_q = DeviceBuffer.of(q)
+        offs = np.concatenate([[0], np.cumsum(ks)]).astype(np.int64)
+        d_out = DeviceBuffer(int(offs[-1]) * NEIGHBOR.itemsize)
+        d_cnt = DeviceBuffer(4 * n)
+        fbufs, frefs = [], (FilterRef * max(1, len(filters)))()
+        for i, f in enumerate(filters):
+            w, size = filter_words(f)
+            b = DeviceBuffer.of(w if w.size else np.zeros(1, np.uint64))
+            fbufs.append(b)
+            frefs[i] = FilterRef(b.ptr, size)
+        tasks = (Task * n)()
+        for i in range(n):
+            tasks[i] = Task(d_q.ptr + i * dim * 4, d_out.ptr + int(offs[i]) * NEIGHBOR.itemsize, d_cnt.ptr + 4 * i,
+                            ks[i], efs[i], fids[i], 0)
+        stream = Stream()
+        _check(_lib.hnswgpu_search_tasks_device(self._h, C.cast(tasks, C.c_void_p), n, C.cast(frefs, C.c_void_p),
+                                                len(filters), storage, stream.handle))
+        stream.synchronize()
+        rows = d_out.download(np.empty(int(offs[-1]), NEIGHBOR))
+        counts = d_cnt.download(np.empty(n, np.uint32))
+        return [(rows[offs[i]:offs[i + 1]], int(counts[i])) for i in range(n)]
 
     def search_device(self, d_queries: int, nq: int, k: int, ef: int, d_out: int, stream: int = 0,
                       d_filter: int = 0, filter_size: int = 0, d_counts: int = 0) -> None:

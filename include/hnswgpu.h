@@ -107,6 +107,32 @@ int hnswgpu_search_batch_device(hnswgpu_index* h, const float* d_qs, uint64_t nq
                                 const uint64_t* d_filter_words, uint64_t filter_size,
                                 hnswgpu_neighbor* d_out, uint32_t* d_counts, void* stream);
 
+/* A mixed batch (no reference counterpart; it generalises search_batch, whose
+ * queries share k, ef and filter): every task is one query with its own k, ef
+ * and filter, and all of them share one launch through a device work queue.
+ * Results are those of search_batch on the same query, k, ef and filter. */
+typedef struct hnswgpu_task {
+  const float* query;        /* device: dim floats */
+  hnswgpu_neighbor* out;     /* device: k rows, padded with {~0, +inf} */
+  uint32_t* count;           /* device: min(k, #results), or NULL */
+  uint32_t k;
+  uint32_t ef;               /* 0 = the index's ef_search; the search uses max(ef, k) */
+  int32_t filter;            /* index into `filters`, or -1 for none */
+  int32_t reserved;
+} hnswgpu_task;
+
+typedef struct hnswgpu_filter {
+  const uint64_t* words;     /* device: BitsetFilter words over labels */
+  uint64_t size;             /* labels covered (BitsetFilter::size) */
+} hnswgpu_filter;
+
+/* Tasks and filters are host arrays of device pointers; asynchronous on
+ * `stream` like hnswgpu_search_batch_device. storage: -1 = the handle's
+ * current "storage" option, else 0 (fp32), 1 (bf16) or 2 (fp8), which must
+ * have been built by setting that option once. */
+int hnswgpu_search_tasks_device(hnswgpu_index* h, const hnswgpu_task* tasks, uint64_t ntasks,
+                                const hnswgpu_filter* filters, uint64_t nfilters, int32_t storage, void* stream);
+
 /* Tuning and test switches (no reference counterpart):
  *   "hash_log2"     visited-hash slots per wavefront (0 = automatic)
  *   "force_spill"   1 = every query through the general (HBM heap) kernel
