@@ -102,6 +102,20 @@ hipError_t launch_search(const DevIndexView& ix, const SearchRequest& r, SearchS
 hipError_t launch_eligibility(const DevIndexView& ix, const uint32_t* deleted_bits, const uint64_t* filter_words,
                               uint64_t filter_size, int by_id, uint32_t* elig, hipStream_t st);
 
+/// One eligibility bitmap of a launch with several filters: filter words by
+/// label (null = deletions only); inactive entries are skipped.
+struct FilterDesc {
+  const uint64_t* words;
+  uint64_t size;
+  uint32_t active;
+  uint32_t pad;
+};
+/// Bitmap f of `elig` (words_per = ceil(count / 32) words each) from descs[f],
+/// f < nf; the grid covers nf, so a launch of fixed shape serves any number of
+/// active filters (graph replays).
+hipError_t launch_eligibility_multi(const DevIndexView& ix, const uint32_t* deleted_bits, const FilterDesc* descs,
+                                    uint32_t nf, uint32_t* elig, hipStream_t st);
+
 /// Largest ef the fast kernel holds in registers.
 constexpr uint32_t kFastMaxEf = 512;
 /// Largest M0 (layer-0 degree) the kernels load in one wave.

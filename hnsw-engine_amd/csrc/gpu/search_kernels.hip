@@ -870,6 +870,28 @@ __global__ void build_eligibility(uint32_t count, const uint64_t* labels, const 
   elig[word] = out;
 }
 
+__global__ void build_eligibility_multi(uint32_t count, const uint64_t* labels, const uint32_t* deleted_bits,
+                                        const FilterDesc* descs, uint32_t* elig) {
+  const FilterDesc d = descs[blockIdx.y];
+  if (d.active == 0u) return;
+  const uint32_t words = (count + 31u) >> 5;
+  const uint32_t word = blockIdx.x * blockDim.x + threadIdx.x;
+  if (word >= words) return;
+  uint32_t out = 0u;
+  const uint32_t del = deleted_bits != nullptr ? deleted_bits[word] : 0u;
+  for (uint32_t b = 0; b < 32; ++b) {
+    const uint32_t id = word * 32u + b;
+    if (id >= count) break;
+    bool ok = ((del >> b) & 1u) == 0u;
+    if (ok && d.words != nullptr) {
+      const uint64_t lab = labels[id];
+      ok = lab < d.size && ((d.words[lab >> 6] >> (lab & 63)) & 1ULL) != 0ULL;
+    }
+    if (ok) out |= 1u << b;
+  }
+  elig[static_cast<size_t>(blockIdx.y) * words + word] = out;
+}
+
 template <int ER, int EC, int PC, int FM, int ST>
 hipError_t run_fast(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
   const size_t lds = static_cast<size_t>(r.waves_per_block) * lds_words_per_wave(r.hash_log2, ix.padded) * 4u;
@@ -971,6 +993,15 @@ hipError_t launch_eligibility(const DevIndexView& ix, const uint32_t* deleted_bi
   if (words == 0) return hipSuccess;
   hipLaunchKernelGGL(build_eligibility, dim3((words + 255) / 256), dim3(256), 0, st, ix.count, ix.labels,
                      deleted_bits, fw, fsize, by_id, elig);
+  return hipGetLastError();
+}
+
+hipError_t launch_eligibility_multi(const DevIndexView& ix, const uint32_t* deleted_bits, const FilterDesc* descs,
+                                    uint32_t nf, uint32_t* elig, hipStream_t st) {
+  const uint32_t words = (ix.count + 31u) >> 5;
+  if (words == 0 || nf == 0) return hipSuccess;
+  hipLaunchKernelGGL(build_eligibility_multi, dim3((words + 255) / 256, nf), dim3(256), 0, st, ix.count, ix.labels,
+                     deleted_bits, descs, elig);
   return hipGetLastError();
 }
 

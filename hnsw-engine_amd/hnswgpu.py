@@ -77,6 +77,12 @@ def _load():
         "hnsw_brute_force_add": ([p, p, p, u64], C.c_int),
         "hnsw_brute_force_search_batch": ([p, p, u64, u64, p, u64, p, u64], C.c_int),
         "hnswgpu_search_tasks_device": ([p, p, u64, p, u64, i32, p], C.c_int),
+        "hnswgpu_server_create": ([p, p, p], C.c_int),
+        "hnswgpu_server_submit": ([p, p, p], C.c_int),
+        "hnswgpu_server_wait": ([p, u64], C.c_int),
+        "hnswgpu_server_stats": ([p, p, u64], C.c_int),
+        "hnswgpu_server_reset_stats": ([p], C.c_int),
+        "hnswgpu_server_free": ([p], C.c_int),
         "hnsw_brute_force_set_option": ([p, C.c_char_p, C.c_int64], C.c_int),
         "hnsw_brute_force_get_option": ([p, C.c_char_p, p], C.c_int),
     }
@@ -225,6 +231,78 @@ class Task(C.Structure):
 class FilterRef(C.Structure):
     """hnswgpu_filter: BitsetFilter words on the device and their label count."""
     _fields_ = [("words", C.c_void_p), ("size", C.c_uint64)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("max_batch", C.c_uint32), ("max_rows", C.c_uint64),
+                ("max_filters", C.c_uint32), ("use_graphs", C.c_uint32), ("max_filter_labels", C.c_uint64),
+                ("hbm_budget", C.c_uint64), ("spill_waves", C.c_uint32), ("storages", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("queries", C.c_void_p), ("nq", C.c_uint64), ("k", C.c_uint64), ("ef", C.c_uint64),
+                ("filter_words", C.c_void_p), ("filter_size", C.c_uint64), ("storage", C.c_int32),
+                ("reserved", C.c_int32), ("out", C.c_void_p), ("counts", C.c_void_p)]
+
+
+SERVER_STATS = ["requests", "queries", "launches", "mean_batch", "p50_ms", "p99_ms", "max_ms", "qps", "mean_ms",
+                "slots", "slot_bytes", "graph_launches"]
+
+
+class Server:
+    """Continuous-batching serving loop over a resident index (hnswgpu_server_*).
+    submit() queues a request and returns a ticket; wait(ticket) returns its
+    (rows, counts), equal to search_batch on the same arguments."""
+
+    def __init__(self, device: "DeviceIndex", slots: int = 2, max_batch: int = 4096, max_rows: int = 0,
+                 max_filters: int = 8, graphs: bool = True, max_filter_labels: int = 0, hbm_budget: int = 0,
+                 spill_waves: int = 0, storages=()):
+        cfg = ServerConfig(slots, max_batch, max_rows, max_filters, 1 if graphs else 0, max_filter_labels,
+                           hbm_budget, spill_waves, sum(1 << int(st) for st in storages))
+        h = C.c_void_p()
+        _check(_lib.hnswgpu_server_create(device._h, C.byref(cfg), C.byref(h)))
+        self._h, self._device, self._live = h, device, {}
+
+    def submit(self, queries, k: int, ef: int = 0, filter=None, storage: int = -1) -> int:
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim == 1:
+            q = q.reshape(1, -1)
+        nq = q.shape[0]
+        out = np.empty(nq * k, dtype=NEIGHBOR)
+        counts = np.empty(nq, dtype=np.uint32)
+        w, size = filter_words(filter)
+        req = Request(_ptr(q), nq, k, ef, _ptr(w), size, storage, 0, _ptr(out), _ptr(counts))
+        t = C.c_uint64()
+        _check(_lib.hnswgpu_server_submit(self._h, C.byref(req), C.byref(t)))
+        self._live[t.value] = (q, w, out, counts)  # buffers stay alive until wait()
+        return t.value
+
+    def wait(self, ticket: int):
+        rc = _lib.hnswgpu_server_wait(self._h, ticket)
+        q, w, out, counts = self._live.pop(ticket)
+        _check(rc)
+        return out.reshape(q.shape[0], -1), counts
+
+    def stats(self) -> dict:
+        a = np.zeros(len(SERVER_STATS), np.float64)
+        _check(_lib.hnswgpu_server_stats(self._h, _ptr(a), a.size))
+        return dict(zip(SERVER_STATS, a.tolist()))
+
+    def reset_stats(self) -> None:
+        _check(_lib.hnswgpu_server_reset_stats(self._h))
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            self._h = None
+            _check(_lib.hnswgpu_server_free(h))
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceIndex:

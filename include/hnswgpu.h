@@ -133,6 +133,59 @@ typedef struct hnswgpu_filter {
 int hnswgpu_search_tasks_device(hnswgpu_index* h, const hnswgpu_task* tasks, uint64_t ntasks,
                                 const hnswgpu_filter* filters, uint64_t nfilters, int32_t storage, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Serving loop (no reference counterpart; the reference serves search_batch   */
+/* calls from caller threads): continuous batching over a resident index.      */
+/* ------------------------------------------------------------------------ */
+typedef struct hnswgpu_server hnswgpu_server;
+
+typedef struct hnswgpu_server_config {
+  uint32_t slots;              /* launches in flight, each with its own stream and scratch (0 = 2) */
+  uint32_t max_batch;          /* queries per launch (0 = 4096) */
+  uint64_t max_rows;           /* result rows (sum of k) per launch (0 = 64 * max_batch) */
+  uint32_t max_filters;        /* distinct filters per launch (0 = 8) */
+  uint32_t use_graphs;         /* 1 = replay a hipGraph captured per slot and launch shape */
+  uint64_t max_filter_labels;  /* labels a filter may cover (0 = every label of the index) */
+  uint64_t hbm_budget;         /* device bytes all slots may use (0 = a quarter of the free memory);
+                                  the slot count is reduced to fit */
+  uint32_t spill_waves;        /* general-kernel wavefronts per slot (0 = 16) */
+  uint32_t storages;           /* bit s set: build the reduced copy s (1 bf16, 2 fp8) now */
+} hnswgpu_server_config;
+
+/* One request: search_batch's arguments on host buffers, which must stay valid
+ * until hnswgpu_server_wait returns for it. storage: -1 = the index's current
+ * "storage" option, else 0 / 1 / 2 (that copy must be built). */
+typedef struct hnswgpu_request {
+  const float* queries;        /* nq x dim */
+  uint64_t nq;
+  uint64_t k;
+  uint64_t ef;                 /* 0 = ef_search; the search uses max(ef, k) */
+  const uint64_t* filter_words;/* BitsetFilter words over labels, or NULL */
+  uint64_t filter_size;
+  int32_t storage;
+  int32_t reserved;
+  hnswgpu_neighbor* out;       /* nq x k */
+  uint32_t* counts;            /* nq, or NULL */
+} hnswgpu_request;
+
+/* The index must stay uploaded and unchanged (no add, no reload) while a
+ * server uses it; soft deletions are allowed. */
+int hnswgpu_server_create(hnswgpu_index* h, const hnswgpu_server_config* config, hnswgpu_server** out);
+/* Queue a request (never blocks on the device); *ticket identifies it. */
+int hnswgpu_server_submit(hnswgpu_server* s, const hnswgpu_request* request, uint64_t* ticket);
+/* Block until the request has completed; returns its status. Each ticket is
+ * waited for once. Results equal hnswgpu_search_batch on the same arguments. */
+int hnswgpu_server_wait(hnswgpu_server* s, uint64_t ticket);
+/* Statistics since the last reset, as doubles: [0] requests completed,
+ * [1] queries completed, [2] launches, [3] mean queries per launch,
+ * [4] p50 and [5] p99 request latency (ms, submit to completion), [6] max
+ * latency, [7] queries/s from the first submit to the last completion,
+ * [8] mean latency, [9] slots, [10] device bytes per slot, [11] graph replays. */
+int hnswgpu_server_stats(hnswgpu_server* s, double* out, uint64_t n);
+int hnswgpu_server_reset_stats(hnswgpu_server* s);
+/* Completes every submitted request, then stops the server. */
+int hnswgpu_server_free(hnswgpu_server* s);
+
 /* Tuning and test switches (no reference counterpart):
  *   "hash_log2"     visited-hash slots per wavefront (0 = automatic)
  *   "force_spill"   1 = every query through the general (HBM heap) kernel
