@@ -103,6 +103,85 @@ def measured_traffic(ef: int, args) -> dict:
         return {"traffic": None}
 
 
+def serving_bench(ix, Q: np.ndarray, seconds: float) -> dict:
+    """The serving loop under the mixed load of a production index: requests
+    of log-uniform batch size 1..512, k in {1, 10, 100}, ef in {32..512},
+    30 % with a 10 %-density BitsetFilter, half traversing the BF16 copy, 1 %
+    of the base soft-deleted. Closed loop (all requests at once: saturated
+    throughput), then open loop (Poisson arrivals at half that request rate):
+    QPS beside p50 / p99 request latency (submit to completion)."""
+    import queue
+    import threading
+
+    import hnswgpu
+
+    rng = np.random.default_rng(77)
+    n = len(ix)
+    for label in rng.choice(n, n // 100, replace=False):
+        ix.mark_deleted(int(label))
+    dev = ix.device()
+    dev.set_option("storage", 0)
+    fw = hnswgpu.filter_words(rng.random(n) < 0.10)  # one filter shared by the filtered requests
+
+    def make(count):
+        out = []
+        for _ in range(count):
+            nq = int(np.exp(rng.uniform(0.0, np.log(512.0))))
+            s = int(rng.integers(0, Q.shape[0] - nq))
+            out.append((Q[s:s + nq], int(rng.choice([1, 10, 100])), int(rng.choice([32, 64, 128, 256, 512])),
+                        fw if rng.random() < 0.3 else None, 1 if rng.random() < 0.5 else 0))
+        return out
+
+    def summary(st, wall, reqs):
+        nq = sum(r[0].shape[0] for r in reqs)
+        return {"requests": len(reqs), "queries": nq, "wall_s": round(wall, 3), "qps": round(nq / wall, 1),
+                "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
+                "launches": int(st["launches"]), "mean_queries_per_launch": round(st["mean_batch"], 1),
+                "graph_launches": int(st["graph_launches"])}
+
+    srv = hnswgpu.Server(dev, slots=3, max_batch=4096, storages=(1,))
+    warm = make(50)
+    for t in [srv.submit(q, k, ef, f, storage=st) for q, k, ef, f, st in warm]:
+        srv.wait(t)
+    reqs = make(3000)
+    srv.reset_stats()
+    t0 = time.perf_counter()
+    tickets = [srv.submit(q, k, ef, f, storage=st) for q, k, ef, f, st in reqs]
+    for t in tickets:
+        srv.wait(t)
+    closed = summary(srv.stats(), time.perf_counter() - t0, reqs)
+    log(f"serving closed loop {closed}")
+
+    rate = 0.5 * closed["requests"] / closed["wall_s"]
+    reqs = make(max(100, int(rate * seconds)))
+    srv.reset_stats()
+    pending: "queue.Queue[int | None]" = queue.Queue()
+
+    def waiter():
+        while (t := pending.get()) is not None:
+            srv.wait(t)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    due = t0
+    for q, k, ef, f, st in reqs:
+        due += rng.exponential(1.0 / rate)
+        while (now := time.perf_counter()) < due:
+            if due - now > 2e-4:
+                time.sleep(due - now - 1e-4)
+        pending.put(srv.submit(q, k, ef, f, storage=st))
+    pending.put(None)
+    th.join()
+    opened = summary(srv.stats(), time.perf_counter() - t0, reqs)
+    opened["offered_requests_per_s"] = round(rate, 1)
+    log(f"serving open loop {opened}")
+    srv.close()
+    return {"mix": "batch 1..512 log-uniform, k {1,10,100}, ef {32..512}, 30% filtered (10% density), "
+                   "50% bf16 traversal, 1% deleted; 3 slots, max 4096 queries per launch, graphs",
+            "closed_loop": closed, "open_loop_poisson": opened}
+
+
 def recall_at(labels: np.ndarray, truth: np.ndarray, k: int) -> float:
     hits = 0
     for a, b in zip(labels[:, :k], truth[:, :k]):
@@ -168,6 +247,8 @@ def main() -> int:
                     help="device-layer option for every search (e.g. hash_log2=12)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (counter calibration) and exit")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop section")
+    ap.add_argument("--serving-seconds", type=float, default=5.0)
     ap.add_argument("--profile-brute", action="store_true",
                     help="profiling run of the brute force only (n x 128 data, nq queries, k), one line")
     args = ap.parse_args()
@@ -331,6 +412,9 @@ def main() -> int:
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(index_path, Q, truth, k, efs, args.cpu_budget_s)
 
+    # Last: it soft-deletes 1 % of the base.
+    serving = None if args.no_serving else serving_bench(ix, Q, args.serving_seconds)
+
     line = {
         "metric": METRIC,
         "value": round(value, 1) if math.isfinite(value) else None,
@@ -354,6 +438,7 @@ def main() -> int:
                      "bytes_per_query": op["bytes_per_query"]},
         "cpu_baseline": cpu,
         "brute_force": brute,
+        "serving": serving,
         **reduced,
         "latency_ms": {"single_query_p50": round(float(np.percentile(lat, 50)), 4),
                        "single_query_p99": round(float(np.percentile(lat, 99)), 4),

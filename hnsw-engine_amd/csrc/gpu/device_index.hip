@@ -266,6 +266,9 @@ gpu::SearchScratch bind_scratch(hnswgpu_index& h, Scratch& sc, const gpu::Search
   const size_t want = static_cast<size_t>(r.blocks) * r.waves_per_block * table_words * sizeof(uint32_t);
   if (sc.visit_tables.bytes < want) sc.visit_tables_cleared = 0;
   s.visit_tables = static_cast<uint32_t*>(sc.visit_tables.need(want));
+  // Candidate overflow of the filtered search: kRunSlots runs of 64 keys per wavefront.
+  s.cand_runs = static_cast<uint64_t*>(sc.cand_runs.need(static_cast<size_t>(r.blocks) * r.waves_per_block *
+                                                          gpu::kRunSlots * 64 * sizeof(uint64_t)));
   if (sc.visit_tables_cleared < sc.visit_tables.bytes) {
     // Once per allocation; the kernels leave every table empty after each query.
     check(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(sc.visit_tables.p), 0xFFFFFFFFu,

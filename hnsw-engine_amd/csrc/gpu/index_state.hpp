@@ -85,7 +85,7 @@ struct PinnedBuf {
 /// need separate Scratch objects (each holds the visited tables of the
 /// wavefronts of its kernel).
 struct Scratch {
-  DevBuf counters, overflow, qprep, elig, spill_bits, spill_heaps, visit_tables, stats;
+  DevBuf counters, overflow, qprep, elig, spill_bits, spill_heaps, visit_tables, stats, cand_runs;
   size_t visit_tables_cleared = 0;
   uint64_t spill_alloc_count = 0;
   uint32_t spill_alloc_waves = 0;

@@ -55,7 +55,11 @@ struct SearchScratch {
   uint64_t spill_heap_stride;
   uint32_t spill_waves;
   uint32_t spill_bitmap_words;
+  uint64_t* cand_runs;       // per resident wavefront: kRunSlots x 64 keys (candidate overflow), or null
 };
+
+/// Sorted runs of 64 candidates a wavefront may spill to HBM (filtered search).
+constexpr uint32_t kRunSlots = 128;
 
 /// One query of a mixed launch (device work queue entry): its own k, ef,
 /// eligibility bitmap and output rows, all device pointers.

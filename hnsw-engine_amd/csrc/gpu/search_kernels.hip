@@ -528,14 +528,134 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
   return Handoff::kNone;
 }
 
-/// Layer-0 search with an eligibility bitmap (filter and/or deletions): C and R
-/// are separate sorted arrays; C keeps its 64 * EC smallest entries and the
-/// smallest entry it ever dropped is remembered -- the search stays exact as
-/// long as that entry is never the smallest live candidate, otherwise the
-/// query goes to the general kernel.
+/// Ascending sort of 64 keys, one per lane (bitonic network over the wave).
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint64_t o = from_lane64(v, lane ^ j);
+      const bool up = (lane & k) == 0;
+      const bool low = (lane & j) == 0;
+      v = (low == up) ? (o < v ? o : v) : (o > v ? o : v);
+    }
+  }
+  return v;
+}
+
+/// Candidates the register array C could not hold: an unsorted row of up to
+/// 64 (entry i in lane i) and up to kRunSlots sorted runs of 64 in this
+/// wavefront's HBM region, each consumed from a head position. C and the
+/// overflow together hold every admitted candidate not yet expanded, except
+/// runs found dead (head distance above the worst result of a full R: the
+/// search stops before reaching them), whose slots are reused.
+struct Overflow {
+  uint64_t* runs;   // kRunSlots x 64 keys; null = no overflow (hand the query off instead)
+  uint64_t buf;
+  uint32_t nbuf;
+  uint64_t bufmin;
+  uint64_t head[2];  // lane l: head key of runs l and 64 + l (kKeyInf = free)
+  uint32_t pos[2];
+  uint64_t runmin;
+
+  __device__ __forceinline__ void clear() {
+    buf = kKeyInf;
+    nbuf = 0;
+    bufmin = kKeyInf;
+    head[0] = head[1] = kKeyInf;
+    pos[0] = pos[1] = 0;
+    runmin = kKeyInf;
+  }
+  __device__ __forceinline__ uint64_t min() const { return bufmin < runmin ? bufmin : runmin; }
+  __device__ __forceinline__ void update_runmin() { runmin = wave_min64(head[0] < head[1] ? head[0] : head[1]); }
+
+  /// Sort the row and store it as a run. False when every slot is taken.
+  __device__ bool spill_row(bool r_full, uint32_t tau) {
+    const int lane = lane_id();
+    const uint64_t v = wave_sort64(static_cast<uint32_t>(lane) < nbuf ? buf : kKeyInf);
+    const uint64_t f0 = ballot(head[0] == kKeyInf || (r_full && key_ord(head[0]) > tau));
+    const uint64_t f1 = ballot(head[1] == kKeyInf || (r_full && key_ord(head[1]) > tau));
+    if (f0 == 0 && f1 == 0) return false;
+    const uint32_t slot = f0 != 0 ? static_cast<uint32_t>(__builtin_ctzll(f0))
+                                  : 64u + static_cast<uint32_t>(__builtin_ctzll(f1));
+    runs[static_cast<size_t>(slot) * 64 + lane] = v;
+    wave_sync();
+    const uint64_t h = rdlane64(v, 0);
+    if (static_cast<uint32_t>(lane) == (slot & 63)) {
+      if (slot < 64) {
+        head[0] = h;
+        pos[0] = 0;
+      } else {
+        head[1] = h;
+        pos[1] = 0;
+      }
+    }
+    buf = kKeyInf;
+    nbuf = 0;
+    bufmin = kKeyInf;
+    update_runmin();
+    return true;
+  }
+  /// Append one key (uniform). False when the overflow is full.
+  __device__ __forceinline__ bool push(uint64_t x, bool r_full, uint32_t tau) {
+    if (static_cast<uint32_t>(lane_id()) == nbuf) buf = x;
+    ++nbuf;
+    bufmin = x < bufmin ? x : bufmin;
+    if (nbuf == 64) return spill_row(r_full, tau);
+    return true;
+  }
+  /// The overflow's smallest key precedes C's: move back into C, from the run
+  /// holding it, every key below the smallest key held elsewhere (at least
+  /// that one). Keys C pushes out go back to the overflow. False when full.
+  template <int EC>
+  __device__ bool refill(WaveSorted<EC>& C, uint32_t ccap, uint32_t& sC, bool r_full, uint32_t tau) {
+    const int lane = lane_id();
+    if (nbuf > 0 && bufmin < runmin && !spill_row(r_full, tau)) return false;
+    const uint64_t m0 = ballot(head[0] == runmin), m1 = ballot(head[1] == runmin);
+    const uint32_t slot = m0 != 0 ? static_cast<uint32_t>(__builtin_ctzll(m0))
+                                  : 64u + static_cast<uint32_t>(__builtin_ctzll(m1));
+    const bool mine = static_cast<uint32_t>(lane) == (slot & 63);
+    const uint64_t o0 = (mine && slot < 64) ? kKeyInf : head[0];
+    const uint64_t o1 = (mine && slot >= 64) ? kKeyInf : head[1];
+    const uint64_t other = wave_min64(o0 < o1 ? o0 : o1);
+    const uint64_t bound = other < bufmin ? other : bufmin;
+    const uint32_t p = rdlane(slot < 64 ? pos[0] : pos[1], static_cast<int>(slot & 63));
+    const uint64_t v = runs[static_cast<size_t>(slot) * 64 + lane];
+    const uint32_t cnt = static_cast<uint32_t>(__popcll(ballot(static_cast<uint32_t>(lane) >= p && v < bound)));
+    const uint32_t np = p + cnt;
+    const uint64_t nh = np < 64 ? rdlane64(v, static_cast<int>(np)) : kKeyInf;
+    if (mine) {
+      if (slot < 64) {
+        head[0] = nh;
+        pos[0] = np;
+      } else {
+        head[1] = nh;
+        pos[1] = np;
+      }
+    }
+    update_runmin();
+    for (uint32_t i = 0; i < cnt; ++i) {
+      const uint64_t d = C.insert(rdlane64(v, static_cast<int>(p + i)), ccap);
+      if (d == kKeyInf) {
+        ++sC;
+      } else if (!push(d, r_full, tau)) {
+        return false;
+      }
+    }
+    return true;
+  }
+};
+
+/// Layer-0 search with an eligibility bitmap (filter and/or deletions): C and R/// Layer-0 search with an eligibility bitmap (filter and/or deletions): C and R
+/// are separate sorted arrays; C keeps its 64 * EC smallest entries in
+/// registers and the rest in the overflow, from which entries come back before
+/// they are needed, so candidates are expanded in the exact (distance, id)
+/// order. Without an overflow region, or when it fills, the query goes to the
+/// general kernel.
 template <int ER, int EC, int PC, int ST>
 __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s,
-                                  const WaveLds& w, Visited& vis, uint32_t qi) {
+                                  const WaveLds& w, Visited& vis, uint32_t qi, uint64_t* runs) {
   const int lane = lane_id();
   const uint32_t ccap = 64u * EC;
   const uint32_t ef = a.ef;
@@ -551,7 +671,9 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
   R.clear();
   C.clear();
   uint32_t sR = 0, sC = 0, tau = kOrdInf;
-  uint64_t drop_min = kKeyInf;
+  Overflow ov;
+  ov.runs = runs;
+  ov.clear();
   wave_sync();
   if (lane == 0) hash_visit(w.hash, vis.lds_log2, key_id(ep));
   vis.in_lds = 1;
@@ -568,10 +690,14 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
   Handoff result = Handoff::kNone;
   for (;;) {
     const uint64_t c = C.at(0);
-    if (drop_min < c) {
-      if (sR >= ef && key_ord(drop_min) > tau) break;
-      result = Handoff::kCandidates;
-      break;
+    const uint64_t omin = ov.min();
+    if (omin < c) {
+      if (sR >= ef && key_ord(omin) > tau) break;
+      if (runs == nullptr || !ov.refill<EC>(C, ccap, sC, sR >= ef, tau)) {
+        result = Handoff::kCandidates;
+        break;
+      }
+      continue;
     }
     if (sC == 0) break;
     if (sR >= ef && key_ord(c) > tau) break;
@@ -602,18 +728,26 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
       const uint32_t od = rdlane(od_u, i);
       if (sR >= ef && od >= tau) continue;
       const uint64_t key = key_of(od, rdlane(id_u, i));
-      const uint64_t dropped = C.insert(key, ccap);
-      if (dropped == kKeyInf) {
-        ++sC;
-      } else if (dropped < drop_min) {
-        drop_min = dropped;
-      }
       if (rdlane(el_u, i) != 0u) {
         R.insert(key, ef);
         if (sR < ef) ++sR;
         tau = key_ord(R.at(sR - 1));
       }
+      const uint64_t dropped = C.insert(key, ccap);
+      if (dropped == kKeyInf) {
+        ++sC;
+      } else if (runs != nullptr) {
+        if (!ov.push(dropped, sR >= ef, tau)) {
+          result = Handoff::kCandidates;
+          break;
+        }
+      } else {
+        // No overflow: remember that a candidate was lost; exact while it is never needed.
+        const uint64_t b = ov.bufmin;
+        ov.bufmin = dropped < b ? dropped : b;
+      }
     }
+    if (result != Handoff::kNone) break;
   }
   if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
   vis.finish();
@@ -632,17 +766,17 @@ enum FilterMode : int { kPlain = 0, kFiltered = 1, kPerQuery = 2 };
 /// code of its own class.
 template <int ER, int EC, int PC, int FM, int ST>
 __device__ Handoff run_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s, const WaveLds& w,
-                             Visited& vis, uint32_t qi) {
+                             Visited& vis, uint32_t qi, uint64_t* runs) {
   if constexpr (FM == kPerQuery && ER > 1) {
-    if (a.ef <= 32u * ER) return run_query<ER / 2, EC / 2, PC, FM, ST>(ix, a, s, w, vis, qi);
+    if (a.ef <= 32u * ER) return run_query<ER / 2, EC / 2, PC, FM, ST>(ix, a, s, w, vis, qi, runs);
   }
   if constexpr (FM == kPlain) {
     return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
   } else if constexpr (FM == kFiltered) {
-    return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi);
+    return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi, runs);
   } else {
     if (a.elig == nullptr) return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
-    return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi);
+    return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi, runs);
   }
 }
 
@@ -658,6 +792,7 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView 
   Visited vis{w.hash, r.hash_log2,
               s.visit_tables != nullptr ? s.visit_tables + (wave_slot << s.visit_log2) : nullptr,
               s.visit_log2, 0u, 0u, false};
+  uint64_t* runs = s.cand_runs != nullptr ? s.cand_runs + wave_slot * (static_cast<size_t>(kRunSlots) * 64) : nullptr;
   const uint32_t nq = launch_nq(r);
   for (;;) {
     uint32_t qi = 0;
@@ -668,7 +803,7 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView 
     if (!r.force_spill) {
       const QueryArgs a = query_args(r, qi);
       // A task whose ef exceeds the register capacity goes to the general kernel.
-      if (a.ef <= 64u * ER) h = run_query<ER, EC, PC, FM, ST>(ix, a, s, w, vis, qi);
+      if (a.ef <= 64u * ER) h = run_query<ER, EC, PC, FM, ST>(ix, a, s, w, vis, qi, runs);
     }
     if (h != Handoff::kNone && lane == 0) {
       const uint32_t slot = atomicAdd(&s.counters[1], 1u);
