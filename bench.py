@@ -248,6 +248,7 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (counter calibration) and exit")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop section")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
     ap.add_argument("--serving-seconds", type=float, default=5.0)
     ap.add_argument("--profile-brute", action="store_true",
                     help="profiling run of the brute force only (n x 128 data, nq queries, k), one line")
@@ -308,9 +309,18 @@ def main() -> int:
     log(f"ground truth in {time.perf_counter() - t0:.1f}s (FP64 check agreement {truth_check}); brute force {brute}")
 
     dev = ix.device()
-    for kv in args.set:
-        name, value = kv.split("=")
-        dev.set_option(name, int(value))
+    tuned = None
+    if args.set or args.no_autotune:
+        for kv in args.set:
+            name, value = kv.split("=")
+            dev.set_option(name, int(value))
+    else:
+        # The autotuner picks the launch shape per ef class on a sample of the queries.
+        t0 = time.perf_counter()
+        dev.autotune(Q[:2000], k, [min(e, 512) for e in efs])
+        tuned = {str(c): {f: dev.get_option(f"tuned_{f}_{c}") for f in ("waves_per_cu", "hash_log2", "generic_gather")}
+                 for c in (64, 128, 256, 512) if dev.get_option(f"tuned_waves_per_cu_{c}") >= 0}
+        log(f"autotune in {time.perf_counter() - t0:.1f}s: {tuned}")
     stream = hnswgpu.Stream()
     d_q = hnswgpu.DeviceBuffer.of(Q)
     d_out = hnswgpu.DeviceBuffer(args.nq * k * hnswgpu.NEIGHBOR.itemsize)
@@ -438,6 +448,7 @@ def main() -> int:
                      "bytes_per_query": op["bytes_per_query"]},
         "cpu_baseline": cpu,
         "brute_force": brute,
+        "autotune": tuned,
         "serving": serving,
         **reduced,
         "latency_ms": {"single_query_p50": round(float(np.percentile(lat, 50)), 4),

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -232,12 +233,25 @@ void build_reduced(hnswgpu_index& h, int storage) {
   red.built = true;
 }
 
+uint32_t ef_class_index(uint32_t ef) {
+  return ef <= 64 ? 0u : ef <= 128 ? 1u : ef <= 256 ? 2u : 3u;
+}
+
 void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq) {
-  r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(std::min<uint32_t>(r.ef, gpu::kFastMaxEf));
+  const uint32_t ef = std::min<uint32_t>(r.ef, gpu::kFastMaxEf);
+  uint32_t waves_per_cu = h.waves_per_cu;
+  r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(ef);
+  r.generic_gather = h.generic_gather;
+  const hnswgpu_index::Tuned& t = h.tuned[ef_class_index(ef)];
+  if (h.use_tuned && t.valid) {
+    waves_per_cu = t.waves_per_cu;
+    r.hash_log2 = t.hash_log2;
+    r.generic_gather = t.generic_gather;
+  }
   const size_t per_wave = static_cast<size_t>(gpu::lds_words_per_wave(r.hash_log2, h.view.padded)) * 4u;
   r.waves_per_block = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / per_wave)));
   // A persistent grid of the resident wavefronts (each owns an HBM visited table).
-  const uint32_t waves = std::min<uint32_t>(resident_waves(per_wave, r.waves_per_block, h.waves_per_cu),
+  const uint32_t waves = std::min<uint32_t>(resident_waves(per_wave, r.waves_per_block, waves_per_cu),
                                             (nq + r.waves_per_block - 1) / r.waves_per_block * r.waves_per_block);
   r.blocks = std::max<uint32_t>(1, waves / r.waves_per_block);
 }
@@ -484,6 +498,81 @@ int hnswgpu_search_tasks_device(hnswgpu_index* h, const hnswgpu_task* tasks, uin
   });
 }
 
+int hnswgpu_autotune(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k, const uint64_t* efs, uint64_t nefs,
+                     char* report, uint64_t report_len) {
+  return guarded([&] {
+    if (h == nullptr || qs == nullptr || efs == nullptr) throw std::invalid_argument("autotune: null pointer");
+    if (nq == 0 || k == 0 || nefs == 0) throw std::invalid_argument("autotune: empty sample");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->view.count == 0) throw std::invalid_argument("autotune: empty index");
+    if (nq * h->dim > (1ull << 31) || nq * k > (1ull << 31)) throw std::invalid_argument("autotune: sample too large");
+    hipStream_t st = h->stream;
+    auto* d_qs = static_cast<float*>(h->host_qs.need(nq * h->dim * sizeof(float)));
+    auto* d_out = static_cast<hnswgpu_neighbor*>(h->host_out.need(nq * k * sizeof(hnswgpu_neighbor)));
+    check(hipMemcpyAsync(d_qs, qs, nq * h->dim * sizeof(float), hipMemcpyHostToDevice, st), "hipMemcpy queries");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    check(hipEventCreate(&e0), "hipEventCreate");
+    check(hipEventCreate(&e1), "hipEventCreate");
+    const auto saved_use = h->use_tuned;
+    const auto saved_timing = h->kernel_timing;
+    h->kernel_timing = 0;
+    std::string text;
+    try {
+      for (uint64_t i = 0; i < nefs; ++i) {
+        const uint64_t ef = std::max<uint64_t>(efs[i], k);
+        if (ef > hnswamd::gpu::kFastMaxEf) continue;  // the general kernel has no launch shape to tune
+        const uint32_t cls = hnswamd::dev::ef_class_index(static_cast<uint32_t>(ef));
+        hnswgpu_index::Tuned best;
+        for (uint32_t wpc : {8u, 12u, 16u, 24u}) {
+          for (uint32_t hl : {11u, 12u, 13u}) {
+            for (int gather : {0, 1}) {
+              if (gather == 1 && h->view.padded != 128) continue;  // no fixed-width loop to compare with
+              hnswgpu_index::Tuned cand;
+              cand.valid = true;
+              cand.waves_per_cu = wpc;
+              cand.hash_log2 = hl;
+              cand.generic_gather = gather;
+              h->tuned[cls] = cand;
+              h->use_tuned = 1;
+              search_device_locked(*h, d_qs, nq, k, ef, nullptr, 0, 0, d_out, nullptr, st);  // warm-up
+              check(hipEventRecord(e0, st), "hipEventRecord");
+              search_device_locked(*h, d_qs, nq, k, ef, nullptr, 0, 0, d_out, nullptr, st);
+              check(hipEventRecord(e1, st), "hipEventRecord");
+              check(hipEventSynchronize(e1), "hipEventSynchronize");
+              float ms = 0.0f;
+              check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+              cand.ms = ms;
+              if (!best.valid || cand.ms < best.ms) best = cand;
+              text += "ef=" + std::to_string(ef) + " waves_per_cu=" + std::to_string(wpc) + " hash_log2=" +
+                      std::to_string(hl) + " generic_gather=" + std::to_string(gather) + " ms=" + std::to_string(ms) +
+                      "\n";
+            }
+          }
+        }
+        h->tuned[cls] = best;
+        text += "best ef=" + std::to_string(ef) + " waves_per_cu=" + std::to_string(best.waves_per_cu) +
+                " hash_log2=" + std::to_string(best.hash_log2) + " generic_gather=" +
+                std::to_string(best.generic_gather) + " ms=" + std::to_string(best.ms) + "\n";
+      }
+    } catch (...) {
+      h->use_tuned = saved_use;
+      h->kernel_timing = saved_timing;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    h->use_tuned = 1;
+    h->kernel_timing = saved_timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (report != nullptr && report_len > 0) {
+      const size_t n = std::min<size_t>(text.size(), report_len - 1);
+      std::memcpy(report, text.data(), n);
+      report[n] = '\0';
+    }
+  });
+}
+
 int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
   return guarded([&] {
     if (h == nullptr || name == nullptr) throw std::invalid_argument("set_option: null pointer");
@@ -508,6 +597,10 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
     } else if (n == "waves_per_cu") {
       if (value < 1 || value > 32) throw std::invalid_argument("waves_per_cu must be 1..32");
       h->waves_per_cu = static_cast<uint32_t>(value);
+    } else if (n == "generic_gather") {
+      h->generic_gather = value != 0 ? 1 : 0;
+    } else if (n == "use_tuned") {
+      h->use_tuned = value != 0 ? 1 : 0;
     } else if (n == "collect_stats") {
       h->collect_stats = value != 0 ? 1 : 0;
     } else if (n == "kernel_timing") {
@@ -574,6 +667,29 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
       float ms = 0.0f;
       check(hipEventElapsedTime(&ms, h->ev_begin, h->ev_end), "hipEventElapsedTime");
       *value = static_cast<int64_t>(static_cast<double>(ms) * 1e6);
+    } else if (n == "generic_gather") {
+      *value = h->generic_gather;
+    } else if (n == "use_tuned") {
+      *value = h->use_tuned;
+    } else if (n.rfind("tuned_", 0) == 0) {
+      // tuned_<field>_<ef class>: the autotuner's choice, or -1 if that class was not tuned.
+      const size_t cut = n.rfind('_');
+      const std::string field = n.substr(6, cut - 6);
+      const int cls = std::atoi(n.c_str() + cut + 1);
+      const int idx = cls == 64 ? 0 : cls == 128 ? 1 : cls == 256 ? 2 : cls == 512 ? 3 : -1;
+      if (idx < 0) throw std::invalid_argument("unknown option " + n);
+      const hnswgpu_index::Tuned& t = h->tuned[idx];
+      if (field == "waves_per_cu") {
+        *value = t.valid ? t.waves_per_cu : -1;
+      } else if (field == "hash_log2") {
+        *value = t.valid ? t.hash_log2 : -1;
+      } else if (field == "generic_gather") {
+        *value = t.valid ? t.generic_gather : -1;
+      } else if (field == "us") {
+        *value = t.valid ? static_cast<int64_t>(t.ms * 1e3) : -1;
+      } else {
+        throw std::invalid_argument("unknown option " + n);
+      }
     } else if (n == "has_dups") {
       *value = h->view.has_dups;
     } else if (n == "num_deleted") {

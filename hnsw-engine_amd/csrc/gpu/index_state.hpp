@@ -116,7 +116,19 @@ struct hnswgpu_index {
   hnswamd::dev::PinnedBuf task_host;
   hipEvent_t task_copied = nullptr;  // the pinned task staging may be reused after this
 
+  // Launch shapes chosen by the autotuner, per ef class (64, 128, 256, 512),
+  // used while use_tuned is set.
+  struct Tuned {
+    bool valid = false;
+    uint32_t waves_per_cu = 16, hash_log2 = 11;
+    int generic_gather = 0;
+    double ms = 0.0;
+  };
+  Tuned tuned[4];
+  int use_tuned = 0;
+
   // options
+  int generic_gather = 0;
   uint32_t spill_waves = 64;
   uint32_t visit_log2 = 15;
   uint32_t waves_per_cu = 16;
@@ -154,6 +166,8 @@ void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq);
 /// Scratch for a launch planned by plan_request with room for nq queries.
 gpu::SearchScratch bind_scratch(hnswgpu_index& h, Scratch& sc, const gpu::SearchRequest& r, uint32_t nq,
                                 uint32_t spill_waves, hipStream_t st);
+/// Index of the capacity class of ef: 0 (<= 64), 1 (<= 128), 2 (<= 256), 3.
+uint32_t ef_class_index(uint32_t ef);
 /// Effective ef of a request: max(ef or ef_search, k).
 uint64_t effective_ef(const hnswgpu_index& h, uint64_t k, uint64_t ef);
 /// Mixed launch of n tasks on `st`: eligibility bitmaps for the filters the

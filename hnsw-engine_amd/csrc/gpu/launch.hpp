@@ -89,6 +89,7 @@ struct SearchRequest {
   uint32_t waves_per_block;
   uint32_t blocks;
   int force_spill;           // route every query through the general kernel
+  int generic_gather;        // 1: the row loop for any padded dim even where a fixed-width one exists
 };
 
 /// Build the reduced copy of the rows for `storage` (1 = bf16, 2 = fp8) into

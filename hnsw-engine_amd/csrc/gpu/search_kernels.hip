@@ -1155,7 +1155,7 @@ hipError_t launch_search(const DevIndexView& ix, const SearchRequest& r, SearchS
     e = hipEventRecord(fast_begin, st);
     if (e != hipSuccess) return e;
   }
-  if (ix.padded == 128) {
+  if (ix.padded == 128 && !r.generic_gather) {
     e = pick_variant<8>(ix, rr, s, st);
   } else {
     e = pick_variant<0>(ix, rr, s, st);

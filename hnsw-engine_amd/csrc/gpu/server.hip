@@ -239,27 +239,48 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
     var += nw * sizeof(uint64_t);
   }
   descs[c.max_filters] = FilterDesc{nullptr, 0, deletions ? 1u : 0u, 0u};
-  for (const Piece& p : slot.pieces) {
-    const hnswgpu_request& r = p.req->r;
-    int fidx = -1;
+  // Fraction of labels each filter allows (its cost grows about as 1 / density).
+  std::vector<double> density(filters.size(), 1.0);
+  for (size_t f = 0; f < filters.size(); ++f) {
+    uint64_t ones = 0;
+    const size_t nw = (filters[f].second + 63) / 64;
+    for (size_t i = 0; i < nw; ++i) ones += static_cast<uint64_t>(__builtin_popcountll(filters[f].first[i]));
+    density[f] = filters[f].second > 0 ? static_cast<double>(ones) / static_cast<double>(filters[f].second) : 1.0;
+  }
+  std::vector<int> fidx(slot.pieces.size(), -1);
+  std::vector<double> cost(slot.pieces.size());
+  for (size_t pi = 0; pi < slot.pieces.size(); ++pi) {
+    const hnswgpu_request& r = slot.pieces[pi].req->r;
     for (size_t f = 0; f < filters.size(); ++f) {
-      if (filters[f].first == r.filter_words && filters[f].second == r.filter_size) fidx = static_cast<int>(f);
+      if (filters[f].first == r.filter_words && filters[f].second == r.filter_size) fidx[pi] = static_cast<int>(f);
     }
-    const uint32_t* e = fidx >= 0 ? elig + static_cast<size_t>(fidx) * srv.words
-                                  : (deletions ? elig + static_cast<size_t>(c.max_filters) * srv.words : nullptr);
+    cost[pi] = static_cast<double>(hnswamd::dev::effective_ef(h, r.k, r.ef)) /
+               (fidx[pi] >= 0 ? std::max(density[fidx[pi]], 0.01) : 1.0);
+  }
+  // The work queue runs the costliest queries first, so that no long query
+  // starts at the end of the launch (longest processing time first). Each
+  // query keeps its output slots.
+  std::vector<size_t> order(slot.pieces.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  uint32_t next_task = 0;
+  for (size_t pi : order) {
+    const Piece& p = slot.pieces[pi];
+    const hnswgpu_request& r = p.req->r;
+    const uint32_t* e = fidx[pi] >= 0 ? elig + static_cast<size_t>(fidx[pi]) * srv.words
+                                      : (deletions ? elig + static_cast<size_t>(c.max_filters) * srv.words : nullptr);
     const uint64_t ef = hnswamd::dev::effective_ef(h, r.k, r.ef);
     const size_t qbytes = srv.dim * sizeof(float);
     std::memcpy(base + var, r.queries + p.first * srv.dim, p.n * qbytes);
     for (uint32_t i = 0; i < p.n; ++i) {
-      const uint32_t t = p.task + i;
       QueryTask q{};
       q.query = reinterpret_cast<const float*>(dbase + var + i * qbytes);
       q.out = reinterpret_cast<hnswamd::gpu::OutRow*>(dout + srv.counts_bytes) + p.row + i * r.k;
-      q.count = reinterpret_cast<uint32_t*>(dout) + t;
+      q.count = reinterpret_cast<uint32_t*>(dout) + p.task + i;
       q.elig = e;
       q.k = static_cast<uint32_t>(r.k);
       q.ef = static_cast<uint32_t>(ef);
-      tasks[t] = q;
+      tasks[next_task++] = q;
     }
     var += p.n * qbytes;
   }

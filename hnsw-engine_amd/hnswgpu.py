@@ -77,6 +77,7 @@ def _load():
         "hnsw_brute_force_add": ([p, p, p, u64], C.c_int),
         "hnsw_brute_force_search_batch": ([p, p, u64, u64, p, u64, p, u64], C.c_int),
         "hnswgpu_search_tasks_device": ([p, p, u64, p, u64, i32, p], C.c_int),
+        "hnswgpu_autotune": ([p, p, u64, u64, p, u64, p, u64], C.c_int),
         "hnswgpu_server_create": ([p, p, p], C.c_int),
         "hnswgpu_server_submit": ([p, p, p], C.c_int),
         "hnswgpu_server_wait": ([p, u64], C.c_int),
@@ -341,6 +342,15 @@ class DeviceIndex:
         out = np.empty((nq, 3), dtype=np.uint32)
         _check(_lib.hnswgpu_read_stats(self._h, _ptr(out), nq))
         return out
+
+    def autotune(self, queries, k: int, efs) -> str:
+        """Time every launch shape per ef class on these queries and keep the
+        fastest (hnswgpu_autotune); returns the measurement report."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        e = np.ascontiguousarray(efs, dtype=np.uint64)
+        buf = C.create_string_buffer(1 << 16)
+        _check(_lib.hnswgpu_autotune(self._h, _ptr(q), q.shape[0], k, _ptr(e), e.size, buf, len(buf)))
+        return buf.value.decode()
 
     def search_mixed(self, queries, ks, efs, filter_ids=None, filters=(), storage: int = -1):
         """One launch for queries with their own k, ef and filter (a filter id

@@ -186,15 +186,35 @@ int hnswgpu_server_reset_stats(hnswgpu_server* s);
 /* Completes every submitted request, then stops the server. */
 int hnswgpu_server_free(hnswgpu_server* s);
 
+/* Autotuner (no reference counterpart): for each ef in efs (one per capacity
+ * class 64 / 128 / 256 / 512 suffices), times the search of the nq sample
+ * queries (host, k results) with HIP events for every launch shape --
+ * wavefronts per CU {8, 12, 16, 24} x LDS visited-hash slots 2^{11, 12, 13} x
+ * row gather (fixed-width loop for padded dim 128, or the generic one) -- and
+ * keeps the fastest per class; searches then use it ("use_tuned" = 1). One
+ * wavefront serves one query in every shape. Results do not depend on the
+ * shape. report (nullable) receives one line per measurement. */
+int hnswgpu_autotune(hnswgpu_index* h, const float* queries, uint64_t nq, uint64_t k, const uint64_t* efs,
+                     uint64_t nefs, char* report, uint64_t report_len);
+
 /* Tuning and test switches (no reference counterpart):
- *   "hash_log2"     visited-hash slots per wavefront (0 = automatic)
- *   "force_spill"   1 = every query through the general (HBM heap) kernel
- *   "spill_waves"   wavefronts of the general kernel
- *   "collect_stats" 1 = record per-query work counts (hnswgpu_read_stats)
- *   "kernel_timing" 1 = bracket the fast kernel with events on its stream
- * and read-only values of the last search:
+ *   "hash_log2"      visited-hash slots per wavefront (0 = automatic)
+ *   "waves_per_cu"   resident wavefronts per CU of the persistent grid
+ *   "visit_log2"     slots of each wavefront's HBM visited table
+ *   "generic_gather" 1 = the generic row loop even for padded dim 128
+ *   "use_tuned"      1 = the autotuner's launch shapes (set by hnswgpu_autotune)
+ *   "storage"        traversal copy, below
+ *   "force_spill"    1 = every query through the general (HBM heap) kernel
+ *   "spill_waves"    wavefronts of the general kernel
+ *   "collect_stats"  1 = record per-query work counts (hnswgpu_read_stats)
+ *   "kernel_timing"  1 = bracket the fast kernel with events on its stream
+ * and read-only values:
  *   "last_overflow"  queries the fast kernel handed to the general kernel
- *   "last_kernel_ns" fast-kernel time (needs kernel_timing; waits for it). */
+ *   "last_hbm_visited", "last_handoff_candidates", "last_handoff_visited"
+ *   "last_kernel_ns" fast-kernel time (needs kernel_timing; waits for it)
+ *   "tuned_waves_per_cu_<c>", "tuned_hash_log2_<c>", "tuned_generic_gather_<c>",
+ *   "tuned_us_<c>"   the autotuner's choice for ef class c in {64, 128, 256, 512}
+ *                    (-1 = not tuned). */
 int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value);
 int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value);
 
