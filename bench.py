@@ -315,9 +315,9 @@ def main() -> int:
             name, value = kv.split("=")
             dev.set_option(name, int(value))
     else:
-        # The autotuner picks the launch shape per ef class on a sample of the queries.
+        # The autotuner picks the launch shape per ef class on batches of the size timed below.
         t0 = time.perf_counter()
-        dev.autotune(Q[:2000], k, [min(e, 512) for e in efs])
+        dev.autotune(Q, k, [min(e, 512) for e in efs])
         tuned = {str(c): {f: dev.get_option(f"tuned_{f}_{c}") for f in ("waves_per_cu", "hash_log2", "generic_gather")}
                  for c in (64, 128, 256, 512) if dev.get_option(f"tuned_waves_per_cu_{c}") >= 0}
         log(f"autotune in {time.perf_counter() - t0:.1f}s: {tuned}")

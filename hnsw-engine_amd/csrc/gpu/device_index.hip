@@ -680,11 +680,11 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
       if (idx < 0) throw std::invalid_argument("unknown option " + n);
       const hnswgpu_index::Tuned& t = h->tuned[idx];
       if (field == "waves_per_cu") {
-        *value = t.valid ? t.waves_per_cu : -1;
+        *value = t.valid ? static_cast<int64_t>(t.waves_per_cu) : -1;
       } else if (field == "hash_log2") {
-        *value = t.valid ? t.hash_log2 : -1;
+        *value = t.valid ? static_cast<int64_t>(t.hash_log2) : -1;
       } else if (field == "generic_gather") {
-        *value = t.valid ? t.generic_gather : -1;
+        *value = t.valid ? static_cast<int64_t>(t.generic_gather) : -1;
       } else if (field == "us") {
         *value = t.valid ? static_cast<int64_t>(t.ms * 1e3) : -1;
       } else {

@@ -193,7 +193,10 @@ int hnswgpu_server_free(hnswgpu_server* s);
  * row gather (fixed-width loop for padded dim 128, or the generic one) -- and
  * keeps the fastest per class; searches then use it ("use_tuned" = 1). One
  * wavefront serves one query in every shape. Results do not depend on the
- * shape. report (nullable) receives one line per measurement. */
+ * shape. The best shape depends on the batch size (a batch smaller than the
+ * resident wavefronts is not bound by occupancy), so the sample should be as
+ * large as the batches to be served. report (nullable) receives one line per
+ * measurement. */
 int hnswgpu_autotune(hnswgpu_index* h, const float* queries, uint64_t nq, uint64_t k, const uint64_t* efs,
                      uint64_t nefs, char* report, uint64_t report_len);
 

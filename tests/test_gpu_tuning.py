@@ -25,7 +25,7 @@ def test_autotune_choices_and_exactness(lib):
     before = {ef: ix.search_rows(Q, 10, ef) for ef in (32, 100, 200, 400)}
     filt = np.random.default_rng(2).random(X.shape[0]) < 0.2
     before_f = ix.search_rows(Q, 10, 150, filter=filt)
-    report = dev.autotune(Q[:400], 10, [32, 100, 200, 400])
+    report = dev.autotune(Q, 10, [32, 100, 200, 400])
     assert report.count("best ef=") == 4
     for cls in (64, 128, 256, 512):
         assert dev.get_option(f"tuned_waves_per_cu_{cls}") in (8, 12, 16, 24)
