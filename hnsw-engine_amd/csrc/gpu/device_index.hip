@@ -242,6 +242,7 @@ void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq) {
   uint32_t waves_per_cu = h.waves_per_cu;
   r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(ef);
   r.generic_gather = h.generic_gather;
+  r.wide_hash = h.wide_hash;
   const hnswgpu_index::Tuned& t = h.tuned[ef_class_index(ef)];
   if (h.use_tuned && t.valid) {
     waves_per_cu = t.waves_per_cu;
@@ -601,6 +602,8 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
       h->generic_gather = value != 0 ? 1 : 0;
     } else if (n == "use_tuned") {
       h->use_tuned = value != 0 ? 1 : 0;
+    } else if (n == "wide_hash") {
+      h->wide_hash = value != 0 ? 1 : 0;
     } else if (n == "collect_stats") {
       h->collect_stats = value != 0 ? 1 : 0;
     } else if (n == "kernel_timing") {
@@ -671,6 +674,8 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
       *value = h->generic_gather;
     } else if (n == "use_tuned") {
       *value = h->use_tuned;
+    } else if (n == "wide_hash") {
+      *value = h->wide_hash;
     } else if (n.rfind("tuned_", 0) == 0) {
       // tuned_<field>_<ef class>: the autotuner's choice, or -1 if that class was not tuned.
       const size_t cut = n.rfind('_');

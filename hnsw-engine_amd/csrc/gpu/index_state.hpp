@@ -129,6 +129,7 @@ struct hnswgpu_index {
 
   // options
   int generic_gather = 0;
+  int wide_hash = 0;
   uint32_t spill_waves = 64;
   uint32_t visit_log2 = 15;
   uint32_t waves_per_cu = 16;

@@ -90,6 +90,7 @@ struct SearchRequest {
   uint32_t blocks;
   int force_spill;           // route every query through the general kernel
   int generic_gather;        // 1: the row loop for any padded dim even where a fixed-width one exists
+  int wide_hash;             // 1: 32-bit LDS visited entries even where 16-bit ones hold the ids
 };
 
 /// Build the reduced copy of the rows for `storage` (1 = bf16, 2 = fp8) into

@@ -142,24 +142,79 @@ __device__ __forceinline__ bool hash_has(const uint32_t* tab, uint32_t log2h, ui
   }
 }
 
-/// Exact visited set of one query: an open-addressing hash in LDS and, once
-/// that reaches 3/4 load, a second one in HBM (one table per resident
-/// wavefront, emptied after a query that used it). After the switch new ids go
-/// to the HBM table; lookups check LDS first.
+/// Compact LDS hash: 2^L slots of 16 bits, two per LDS word (0xFFFF empty).
+/// Ids are permuted by an odd multiplier mod 2^B (a bijection on B-bit ids);
+/// the top L bits of the permuted id give the home slot and the entry stores
+/// the other B - L bits with the distance from home, so an entry identifies
+/// its id exactly (linear probing, at most `maxp` probes).
+struct Hash16 {
+  uint32_t L, B, maxp;
+
+  __device__ __forceinline__ void place(uint32_t id, uint32_t& home, uint32_t& rem) const {
+    const uint32_t x = (id * 0x9E3779B1u) & ((1u << B) - 1u);
+    home = x >> (B - L);
+    rem = x & ((1u << (B - L)) - 1u);
+  }
+  /// 1: inserted, 0: present, 2: no free slot within maxp probes.
+  __device__ __forceinline__ int visit(uint32_t* words, uint32_t id) const {
+    uint32_t home, rem;
+    place(id, home, rem);
+    const uint32_t smask = (1u << L) - 1u;
+    for (uint32_t p = 0; p <= maxp; ++p) {
+      const uint32_t slot = (home + p) & smask;
+      const uint32_t want = (p << (B - L)) | rem;
+      uint32_t* wp = words + (slot >> 1);
+      const uint32_t sh = (slot & 1u) << 4;
+      uint32_t cur = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      for (;;) {
+        const uint32_t e = (cur >> sh) & 0xFFFFu;
+        if (e == want) return 0;
+        if (e != 0xFFFFu) break;
+        const uint32_t old = atomicCAS(wp, cur, (cur & ~(0xFFFFu << sh)) | (want << sh));
+        if (old == cur) return 1;
+        cur = old;
+      }
+    }
+    return 2;
+  }
+  __device__ __forceinline__ bool has(const uint32_t* words, uint32_t id) const {
+    uint32_t home, rem;
+    place(id, home, rem);
+    const uint32_t smask = (1u << L) - 1u;
+    for (uint32_t p = 0; p <= maxp; ++p) {
+      const uint32_t slot = (home + p) & smask;
+      const uint32_t e = (words[slot >> 1] >> ((slot & 1u) << 4)) & 0xFFFFu;
+      if (e == ((p << (B - L)) | rem)) return true;
+      if (e == 0xFFFFu) return false;
+    }
+    return false;
+  }
+};
+
+/// Exact visited set of one query: an open-addressing hash in LDS (16-bit
+/// entries when ids fit, else 32-bit) and, once that reaches 3/4 load, a
+/// second one in HBM (one table per resident wavefront, emptied after a query
+/// that used it). After the switch new ids go to the HBM table; lookups check
+/// LDS first.
 struct Visited {
   uint32_t* lds;
-  uint32_t lds_log2;
+  uint32_t lds_log2;  // LDS words
   uint32_t* hbm;  // null: no second level
   uint32_t hbm_log2;
   uint32_t in_lds;
   uint32_t in_hbm;
   bool spilled;
+  bool compact;   // 16-bit LDS entries (h16)
+  Hash16 h16;
+  bool overflow;  // per lane: a compact insert found no slot and went to the HBM table
+  bool lost;      // per lane: ... and there is no HBM table
 
+  __device__ __forceinline__ uint32_t lds_slots() const { return 1u << (compact ? lds_log2 + 1 : lds_log2); }
   /// Room for `more` new ids? Switches to the HBM table when the LDS one is
   /// full; false when both are.
   __device__ __forceinline__ bool reserve(uint32_t more) {
     if (!spilled) {
-      if (in_lds + more <= ((1u << lds_log2) >> 2) * 3) return true;
+      if (in_lds + more <= (lds_slots() >> 2) * 3) return true;
       if (hbm == nullptr) return false;
       spilled = true;
     }
@@ -167,9 +222,28 @@ struct Visited {
   }
   /// Per-lane test-and-set; every lane holding an id calls it together.
   __device__ __forceinline__ bool visit(uint32_t id) {
-    if (!spilled) return hash_visit(lds, lds_log2, id);
-    if (hash_has(lds, lds_log2, id)) return false;
+    if (!spilled) {
+      if (!compact) return hash_visit(lds, lds_log2, id);
+      const int r = h16.visit(lds, id);
+      if (r != 2) return r == 1;
+      overflow = true;
+      if (hbm == nullptr) {
+        lost = true;
+        return false;
+      }
+      return hash_visit(hbm, hbm_log2, id);
+    }
+    if (compact ? h16.has(lds, id) : hash_has(lds, lds_log2, id)) return false;
     return hash_visit(hbm, hbm_log2, id);
+  }
+  /// After a batch of visits: a compact-hash overflow moves the set to its
+  /// HBM level (the overflowed ids are there). False if an id had nowhere to go.
+  __device__ __forceinline__ bool settle() {
+    const bool any_overflow = ballot(overflow) != 0, any_lost = ballot(lost) != 0;
+    overflow = false;
+    lost = false;
+    if (any_overflow) spilled = true;
+    return !any_lost;
   }
   __device__ __forceinline__ void count(uint32_t fresh) {
     if (spilled) {
@@ -445,7 +519,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
   for (int e = 0; e < ER; ++e) fl[e] = 0u;
   uint32_t ties = 0u, n_ties = 0u;
   wave_sync();
-  if (lane == 0) hash_visit(w.hash, vis.lds_log2, key_id(ep));
+  if (lane == 0) (void)vis.visit(key_id(ep));
   vis.in_lds = 1;
   wave_sync();
   uint32_t dummy;
@@ -480,6 +554,10 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
     if (peek != ~0u) next = load_list0(ix, key_id(R.at(peek)));
 
     const uint32_t U = fresh_from_list(ix, w, cur, nullptr, [&](uint32_t id) { return vis.visit(id); });
+    if (vis.compact && !vis.settle()) {
+      result = Handoff::kVisited;
+      break;
+    }
     vis.count(U);
     work.dist += U;
     ++work.exp0;
@@ -675,7 +753,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
   ov.runs = runs;
   ov.clear();
   wave_sync();
-  if (lane == 0) hash_visit(w.hash, vis.lds_log2, key_id(ep));
+  if (lane == 0) (void)vis.visit(key_id(ep));
   vis.in_lds = 1;
   wave_sync();
   C.insert(ep, ccap);
@@ -713,6 +791,10 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
     if (peek != kKeyInf) next = load_list0(ix, key_id(peek));
 
     const uint32_t U = fresh_from_list(ix, w, cur, a.elig, [&](uint32_t id) { return vis.visit(id); });
+    if (vis.compact && !vis.settle()) {
+      result = Handoff::kVisited;
+      break;
+    }
     vis.count(U);
     work.dist += U;
     ++work.exp0;
@@ -789,9 +871,15 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView 
   uint32_t* base = lds + static_cast<size_t>(threadIdx.x >> 6) * per_wave;
   const WaveLds w = wave_lds(base, r.hash_log2, ix.padded);
   const size_t wave_slot = static_cast<size_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // 16-bit LDS entries when an entry can hold the id bits beyond the home
+  // slot (B - L <= 8) with 8 bits of probe distance; else 32-bit entries.
+  const uint32_t L = r.hash_log2 + 1u;
+  uint32_t B = ix.count > 1u ? 32u - static_cast<uint32_t>(__builtin_clz(ix.count - 1u)) : 1u;
+  B = B < L ? L : B;
+  const bool compact = r.wide_hash == 0 && B <= L + 8u;
   Visited vis{w.hash, r.hash_log2,
               s.visit_tables != nullptr ? s.visit_tables + (wave_slot << s.visit_log2) : nullptr,
-              s.visit_log2, 0u, 0u, false};
+              s.visit_log2, 0u, 0u, false, compact, Hash16{L, B, 254u}, false, false};
   uint64_t* runs = s.cand_runs != nullptr ? s.cand_runs + wave_slot * (static_cast<size_t>(kRunSlots) * 64) : nullptr;
   const uint32_t nq = launch_nq(r);
   for (;;) {

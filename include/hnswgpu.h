@@ -205,6 +205,7 @@ int hnswgpu_autotune(hnswgpu_index* h, const float* queries, uint64_t nq, uint64
  *   "waves_per_cu"   resident wavefronts per CU of the persistent grid
  *   "visit_log2"     slots of each wavefront's HBM visited table
  *   "generic_gather" 1 = the generic row loop even for padded dim 128
+ *   "wide_hash"      1 = 32-bit LDS visited entries (default: 16-bit when ids fit)
  *   "use_tuned"      1 = the autotuner's launch shapes (set by hnswgpu_autotune)
  *   "storage"        traversal copy, below
  *   "force_spill"    1 = every query through the general (HBM heap) kernel

@@ -73,12 +73,16 @@ def test_visited_set_spills_to_hbm_and_stays_exact(golden, lib, oracle):
     Q = golden["l2_d32/queries"]
     want = orc.search(Q, 10, 64, order="avx512")
     dev = ix.device()
-    # A 256-slot LDS hash: queries continue in their HBM tables.
-    dev.set_option("hash_log2", 8)
-    rows, _ = dev.search_rows(Q, 10, 64)
-    assert dev.get_option("last_hbm_visited") > 0
-    assert dev.get_option("last_overflow") == 0
-    assert rows_equal(rows, want)
+    # A 256-word LDS hash (256 32-bit slots, or 512 16-bit ones): queries
+    # continue in their HBM tables.
+    for wide, ef in ((1, 64), (0, 200)):
+        dev.set_option("wide_hash", wide)
+        dev.set_option("hash_log2", 8)
+        rows, _ = dev.search_rows(Q, 10, ef)
+        assert dev.get_option("last_hbm_visited") > 0, wide
+        assert dev.get_option("last_overflow") == 0
+        assert rows_equal(rows, orc.search(Q, 10, ef, order="avx512")), wide
+    dev.set_option("wide_hash", 1)
     # A 256-slot HBM table as well: the rest go to the general kernel.
     dev.set_option("visit_log2", 8)
     rows, _ = dev.search_rows(Q, 10, 64)
@@ -86,8 +90,10 @@ def test_visited_set_spills_to_hbm_and_stays_exact(golden, lib, oracle):
     assert rows_equal(rows, want)
     dev.set_option("visit_log2", 15)
     dev.set_option("hash_log2", 0)
-    rows, _ = dev.search_rows(Q, 10, 64)
-    assert rows_equal(rows, want)
+    for wide in (1, 0):
+        dev.set_option("wide_hash", wide)
+        rows, _ = dev.search_rows(Q, 10, 64)
+        assert rows_equal(rows, want)
 
 
 def test_uniform_20k_x_128_parity(lib, oracle, tmp_path):
