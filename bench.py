@@ -310,7 +310,7 @@ def main() -> int:
 
     dev = ix.device()
     tuned = None
-    if args.set or args.no_autotune:
+    if args.set or args.no_autotune or args.profile:  # a profile times the default launch shapes
         for kv in args.set:
             name, value = kv.split("=")
             dev.set_option(name, int(value))

@@ -124,11 +124,18 @@ void upload_view(hnswgpu_index& h, const hnswgpu_index_view& v) {
   h.upper_used = v.upper_used;
 }
 
-/// LDS visited-hash size: about 16 slots per unit of ef, within [2^11, 2^13];
-/// queries that visit more continue in the HBM table.
-uint32_t auto_hash_log2(uint32_t ef) {
-  const uint32_t l = next_log2(static_cast<uint64_t>(ef) * 16);
-  return std::min<uint32_t>(13, std::max<uint32_t>(11, l));
+/// LDS visited-hash words: about 16 slots per unit of ef, within [2^11, 2^13]
+/// words; queries that visit more continue in the HBM table. With 16-bit
+/// entries (two slots per word; ids of at most 2^(log2 slots + 8) nodes) that
+/// is 8 words per unit of ef.
+uint32_t auto_hash_log2(uint32_t ef, uint32_t count, bool wide) {
+  auto clamp = [](uint32_t l) { return std::min<uint32_t>(13, std::max<uint32_t>(11, l)); };
+  if (!wide) {
+    const uint32_t l = clamp(next_log2(static_cast<uint64_t>(ef) * 8));
+    const uint32_t id_bits = count > 1 ? next_log2(count) : 1;
+    if (id_bits <= l + 1 + 8) return l;
+  }
+  return clamp(next_log2(static_cast<uint64_t>(ef) * 16));
 }
 
 /// Wavefronts resident at once for a kernel using `lds_per_wave` bytes of LDS
@@ -240,7 +247,7 @@ uint32_t ef_class_index(uint32_t ef) {
 void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq) {
   const uint32_t ef = std::min<uint32_t>(r.ef, gpu::kFastMaxEf);
   uint32_t waves_per_cu = h.waves_per_cu;
-  r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(ef);
+  r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(ef, h.view.count, h.wide_hash != 0);
   r.generic_gather = h.generic_gather;
   r.wide_hash = h.wide_hash;
   const hnswgpu_index::Tuned& t = h.tuned[ef_class_index(ef)];
@@ -524,8 +531,8 @@ int hnswgpu_autotune(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k,
         if (ef > hnswamd::gpu::kFastMaxEf) continue;  // the general kernel has no launch shape to tune
         const uint32_t cls = hnswamd::dev::ef_class_index(static_cast<uint32_t>(ef));
         hnswgpu_index::Tuned best;
-        for (uint32_t wpc : {8u, 12u, 16u, 24u}) {
-          for (uint32_t hl : {11u, 12u, 13u}) {
+        for (uint32_t wpc : {8u, 12u, 16u, 20u, 24u}) {
+          for (uint32_t hl : {10u, 11u, 12u, 13u}) {
             for (int gather : {0, 1}) {
               if (gather == 1 && h->view.padded != 128) continue;  // no fixed-width loop to compare with
               hnswgpu_index::Tuned cand;

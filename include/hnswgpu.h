@@ -189,7 +189,7 @@ int hnswgpu_server_free(hnswgpu_server* s);
 /* Autotuner (no reference counterpart): for each ef in efs (one per capacity
  * class 64 / 128 / 256 / 512 suffices), times the search of the nq sample
  * queries (host, k results) with HIP events for every launch shape --
- * wavefronts per CU {8, 12, 16, 24} x LDS visited-hash slots 2^{11, 12, 13} x
+ * wavefronts per CU {8, 12, 16, 20, 24} x LDS visited-hash words 2^{10..13} x
  * row gather (fixed-width loop for padded dim 128, or the generic one) -- and
  * keeps the fastest per class; searches then use it ("use_tuned" = 1). One
  * wavefront serves one query in every shape. Results do not depend on the

@@ -28,8 +28,8 @@ def test_autotune_choices_and_exactness(lib):
     report = dev.autotune(Q, 10, [32, 100, 200, 400])
     assert report.count("best ef=") == 4
     for cls in (64, 128, 256, 512):
-        assert dev.get_option(f"tuned_waves_per_cu_{cls}") in (8, 12, 16, 24)
-        assert dev.get_option(f"tuned_hash_log2_{cls}") in (11, 12, 13)
+        assert dev.get_option(f"tuned_waves_per_cu_{cls}") in (8, 12, 16, 20, 24)
+        assert dev.get_option(f"tuned_hash_log2_{cls}") in (10, 11, 12, 13)
         assert dev.get_option(f"tuned_us_{cls}") > 0
     assert dev.get_option("use_tuned") == 1
     for ef, want in before.items():
