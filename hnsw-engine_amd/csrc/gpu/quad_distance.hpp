@@ -79,5 +79,73 @@ __device__ __forceinline__ float quad_raw(bool l2, const float* __restrict__ row
   return (v[0] + v[2]) + (v[1] + v[3]);
 }
 
+/// quad_raw's sum for a padded row of whole chunks only (no partial chunk),
+/// with the metric fixed at compile time. All of a row's loads are issued
+/// before any arithmetic (PC chunks at once, or groups of 8 when the chunk
+/// count pc is only known at run time), so a row costs one memory round trip.
+/// The summation order is quad_raw's: chunk c of a full 64-float block goes to
+/// accumulator c mod 4, later chunks to accumulator 0, each accumulator in
+/// increasing chunk order. Inactive quads return 0.
+template <int PC, bool L2>
+__device__ __forceinline__ float quad_raw_rows(const float* __restrict__ row, const float* q, uint32_t pc_rt,
+                                               bool active) {
+  constexpr int kGroup = PC > 0 ? PC : 8;
+  const uint32_t pc = PC > 0 ? static_cast<uint32_t>(PC) : pc_rt;
+  const uint32_t full = (pc >> 2) << 2;
+  const uint32_t g4 = static_cast<uint32_t>(lane_id() & 3) * 4;
+  float a[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) a[i][e] = 0.0f;
+  }
+  for (uint32_t c0 = 0; c0 < pc; c0 += kGroup) {
+    float4 x[kGroup];
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) x[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < kGroup; ++j) {
+        if (PC > 0 || c0 + j < pc) x[j] = *reinterpret_cast<const float4*>(row + 16 * (c0 + j) + g4);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) {
+      const uint32_t c = c0 + static_cast<uint32_t>(j);
+      if (PC == 0 && c >= pc) break;
+      // The query is read from LDS four chunks at a time: the first four chunks' sums are
+      // complete before the next loads, so fewer registers are live at once.
+      if (j > 0 && (j & 3) == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) asm volatile("" : "+v"(a[i][e])::"memory");
+        }
+      }
+      const float4 y = *reinterpret_cast<const float4*>(q + 16 * c + g4);
+      if (c < full) {
+        // c0 is a multiple of kGroup, which is a multiple of 4 when pc > kGroup: c mod 4 = j mod 4.
+        switch (j & 3) {
+          case 0: acc4(a[0], y, x[j], L2); break;
+          case 1: acc4(a[1], y, x[j], L2); break;
+          case 2: acc4(a[2], y, x[j], L2); break;
+          default: acc4(a[3], y, x[j], L2); break;
+        }
+      } else {
+        acc4(a[0], y, x[j], L2);
+      }
+    }
+  }
+  float v[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) v[e] = (a[0][e] + a[1][e]) + (a[2][e] + a[3][e]);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) v[e] = v[e] + quad_xor2(v[e]);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) v[e] = v[e] + quad_xor1(v[e]);
+  const float r = (v[0] + v[2]) + (v[1] + v[3]);
+  return active ? r : 0.0f;
+}
+
 }  // namespace gpu
 }  // namespace hnswamd

@@ -39,14 +39,36 @@ __device__ __forceinline__ float dot2_bf16(uint32_t a, uint32_t b, float c) {
 
 /// Dot product of a bf16 row with the bf16 query (both `padded` elements),
 /// by the four lanes of a quad: lane g takes 16-byte pieces g, g+4, ...
-/// Identical in all four lanes.
+/// (accumulator 0 the even element pairs of a piece, 1 the odd ones, pieces
+/// in increasing order). Identical in all four lanes. With PC (= padded / 16)
+/// known at compile time every load of the row is issued before the
+/// arithmetic.
+template <int PC>
 __device__ __forceinline__ float quad_dot_bf16(const uint16_t* __restrict__ row, const uint16_t* q,
                                                uint32_t padded, bool active) {
   const uint32_t g = static_cast<uint32_t>(lane_id() & 3);
   float a0 = 0.0f, a1 = 0.0f;
-  if (active) {
-    const uint4* r4 = reinterpret_cast<const uint4*>(row);
-    const uint4* q4 = reinterpret_cast<const uint4*>(q);
+  const uint4* r4 = reinterpret_cast<const uint4*>(row);
+  const uint4* q4 = reinterpret_cast<const uint4*>(q);
+  if constexpr (PC > 0) {
+    static_assert(PC % 2 == 0, "bf16 rows of whole 16-byte pieces per lane");
+    constexpr int kPer = PC / 2;  // 2 PC pieces of 8 elements over 4 lanes
+    uint4 x[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) x[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < kPer; ++i) x[i] = r4[g + 4 * i];
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const uint4 y = q4[g + 4 * i];
+      a0 = dot2_bf16(x[i].x, y.x, a0);
+      a1 = dot2_bf16(x[i].y, y.y, a1);
+      a0 = dot2_bf16(x[i].z, y.z, a0);
+      a1 = dot2_bf16(x[i].w, y.w, a1);
+    }
+  } else if (active) {
     const uint32_t pieces = padded >> 3;
 #pragma unroll 4
     for (uint32_t i = g; i < pieces; i += 4) {
@@ -64,30 +86,47 @@ __device__ __forceinline__ float quad_dot_bf16(const uint16_t* __restrict__ row,
   return v;
 }
 
-/// Dot product of an e4m3 row (before its scale) with the fp32 query.
+__device__ __forceinline__ void fp8_piece(uint4 x, const float* q16, float& a0, float& a1) {
+  const float4* qp = reinterpret_cast<const float4*>(q16);
+  const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float4 y = qp[j];
+    const f32x2_t lo = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(w[j]), false);
+    const f32x2_t hi = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(w[j]), true);
+    a0 = __builtin_fmaf(lo.x, y.x, a0);
+    a1 = __builtin_fmaf(lo.y, y.y, a1);
+    a0 = __builtin_fmaf(hi.x, y.z, a0);
+    a1 = __builtin_fmaf(hi.y, y.w, a1);
+  }
+}
+
+/// Dot product of an e4m3 row (before its scale) with the fp32 query: lane g
+/// of the quad takes 16-element pieces g, g+4, ... With PC (= padded / 16)
+/// known at compile time every load of the row is issued before the
+/// arithmetic.
+template <int PC>
 __device__ __forceinline__ float quad_dot_fp8(const uint8_t* __restrict__ row, const float* q, uint32_t padded,
                                               bool active) {
   const uint32_t g = static_cast<uint32_t>(lane_id() & 3);
   float a0 = 0.0f, a1 = 0.0f;
-  if (active) {
-    const uint4* r4 = reinterpret_cast<const uint4*>(row);
+  const uint4* r4 = reinterpret_cast<const uint4*>(row);
+  if constexpr (PC > 0) {
+    static_assert(PC % 4 == 0, "fp8 rows of whole 16-byte pieces per lane");
+    constexpr int kPer = PC / 4;  // PC pieces of 16 elements over 4 lanes
+    uint4 x[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) x[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < kPer; ++i) x[i] = r4[g + 4 * i];
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) fp8_piece(x[i], q + 16 * (g + 4 * i), a0, a1);
+  } else if (active) {
     const uint32_t pieces = padded >> 4;  // 16 elements per piece
 #pragma unroll 2
-    for (uint32_t i = g; i < pieces; i += 4) {
-      const uint4 x = r4[i];
-      const float4* qp = reinterpret_cast<const float4*>(q + 16 * i);
-      const uint32_t w[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const float4 y = qp[j];
-        const f32x2_t lo = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(w[j]), false);
-        const f32x2_t hi = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(w[j]), true);
-        a0 = __builtin_fmaf(lo.x, y.x, a0);
-        a1 = __builtin_fmaf(lo.y, y.y, a1);
-        a0 = __builtin_fmaf(hi.x, y.z, a0);
-        a1 = __builtin_fmaf(hi.y, y.w, a1);
-      }
-    }
+    for (uint32_t i = g; i < pieces; i += 4) fp8_piece(r4[i], q + 16 * i, a0, a1);
   }
   float v = a0 + a1;
   v = v + quad_xor1(v);

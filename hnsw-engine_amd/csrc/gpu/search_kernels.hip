@@ -80,10 +80,11 @@ __device__ __forceinline__ WaveLds wave_lds(uint32_t* base, uint32_t log2h, uint
 
 /// Ordered distances of the rows w.nid[0..U) into w.nd[0..U), 16 rows per step,
 /// from the fp32 rows (ST = 0, exact) or the bf16 / fp8 copy (ST = 1 / 2).
-template <int PC, int ST = kStoreF32>
-__device__ __forceinline__ void row_dists(const DevIndexView& ix, const WaveLds& w, uint32_t U) {
+/// Every load a row needs (its norm and scale included) is issued before its
+/// arithmetic, so a step costs one memory round trip.
+template <int PC, int ST, bool L2>
+__device__ __forceinline__ void row_dists_metric(const DevIndexView& ix, const WaveLds& w, uint32_t U) {
   const int lane = lane_id();
-  const bool l2 = ix.metric == 0;
   const float qn = ST == kStoreF32 ? 0.0f : w.qn[0];
   for (uint32_t base = 0; base < U; base += 16) {
     const uint32_t r = base + static_cast<uint32_t>(lane >> 2);
@@ -92,20 +93,31 @@ __device__ __forceinline__ void row_dists(const DevIndexView& ix, const WaveLds&
     float d;
     if constexpr (ST == kStoreF32) {
       const float* row = ix.vecs + static_cast<size_t>(id) * ix.padded;
-      d = quad_raw<PC>(l2, row, w.q, ix.padded >> 4, false, active);
-      if (!l2) d = 1.0f - d;
+      d = quad_raw_rows<PC, L2>(row, w.q, ix.padded >> 4, active);
+      if (!L2) d = 1.0f - d;
     } else {
+      const float xn = (L2 && active) ? ix.vnorm[id] : 0.0f;
       float dp;
       if constexpr (ST == kStoreBF16) {
         const uint16_t* row = static_cast<const uint16_t*>(ix.vlow) + static_cast<size_t>(id) * ix.padded;
-        dp = quad_dot_bf16(row, w.qb, ix.padded, active);
+        dp = quad_dot_bf16<PC>(row, w.qb, ix.padded, active);
       } else {
+        const float xs = active ? ix.vscale[id] : 0.0f;
         const uint8_t* row = static_cast<const uint8_t*>(ix.vlow) + static_cast<size_t>(id) * ix.padded;
-        dp = quad_dot_fp8(row, w.q, ix.padded, active) * (active ? ix.vscale[id] : 0.0f);
+        dp = quad_dot_fp8<PC>(row, w.q, ix.padded, active) * xs;
       }
-      d = l2 ? (qn + (active ? ix.vnorm[id] : 0.0f)) - 2.0f * dp : 1.0f - dp;
+      d = L2 ? (qn + xn) - 2.0f * dp : 1.0f - dp;
     }
     if (active && (lane & 3) == 0) w.nd[r] = ord_of(d);
+  }
+}
+
+template <int PC, int ST = kStoreF32>
+__device__ __forceinline__ void row_dists(const DevIndexView& ix, const WaveLds& w, uint32_t U) {
+  if (ix.metric == 0) {
+    row_dists_metric<PC, ST, true>(ix, w, U);
+  } else {
+    row_dists_metric<PC, ST, false>(ix, w, U);
   }
 }
 
@@ -268,27 +280,35 @@ struct Work {
   uint32_t dist = 0, exp0 = 0, expu = 0;
 };
 
-/// Greedy descent from `cur` through layers max_level..1.
+/// Greedy descent from `cur` through layers max_level..1. A list's count and
+/// entries are loaded together, and each neighbour's list offset is fetched
+/// with its row, so a step costs two memory round trips (list, then rows).
 template <int PC, int ST>
 __device__ uint64_t descend(const DevIndexView& ix, const WaveLds& w, uint64_t cur, Work& work) {
   const int lane = lane_id();
+  if (ix.max_level <= 0) return cur;
+  uint32_t cur_off = uniform(ix.upper_off[key_id(cur)]);
   for (int L = ix.max_level; L > 0; --L) {
     for (;;) {
-      const uint32_t cid = key_id(cur);
-      const uint32_t* l = ix.upper + ix.upper_off[cid] + static_cast<size_t>(L - 1) * (1u + ix.M);
-      const uint32_t cnt = uniform(l[0]);
+      const uint32_t* l = ix.upper + cur_off + static_cast<size_t>(L - 1) * (1u + ix.M);
+      const uint32_t head = l[0];
+      const uint32_t nb = static_cast<uint32_t>(lane) < ix.M ? l[1 + lane] : 0u;
+      const uint32_t cnt = uniform(head);
       work.dist += cnt;
       ++work.expu;
       wave_sync();
-      if (static_cast<uint32_t>(lane) < cnt) w.nid[lane] = l[1 + lane];
+      const bool has = static_cast<uint32_t>(lane) < cnt;
+      if (has) w.nid[lane] = nb;
+      const uint32_t nb_off = has ? ix.upper_off[nb] : 0u;
       wave_sync();
       row_dists<PC, ST>(ix, w, cnt);
       wave_sync();
-      const uint64_t mine = static_cast<uint32_t>(lane) < cnt ? key_of(w.nd[lane], static_cast<uint32_t>(lane))
-                                                              : kKeyInf;
+      const uint64_t mine = has ? key_of(w.nd[lane], static_cast<uint32_t>(lane)) : kKeyInf;
       const uint64_t best = wave_min64(mine);
       if (best == kKeyInf || key_ord(best) >= key_ord(cur)) break;
-      cur = key_of(key_ord(best), uniform(w.nid[key_id(best)]));
+      const int at = static_cast<int>(key_id(best));
+      cur = key_of(key_ord(best), rdlane(nb, at));
+      cur_off = rdlane(nb_off, at);
     }
   }
   return cur;
