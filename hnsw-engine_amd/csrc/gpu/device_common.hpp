@@ -59,6 +59,29 @@ __device__ __forceinline__ uint64_t from_lane64(uint64_t v, int src_lane) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// Whole-wave DPP shifts (wave_shr:1 / wave_shl:1): lane i gets the value of
+// lane i - 1 / i + 1, and the lane without a source (0 / 63) gets `fill`. One
+// VALU operation, where a ds_bpermute is a round trip through the LDS crossbar.
+// All 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t fill) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(static_cast<int>(fill), static_cast<int>(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ uint32_t wave_shl1(uint32_t v, uint32_t fill) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(static_cast<int>(fill), static_cast<int>(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ uint64_t wave_shr1_64(uint64_t v, uint64_t fill) {
+  const uint32_t lo = wave_shr1(static_cast<uint32_t>(v), static_cast<uint32_t>(fill));
+  const uint32_t hi = wave_shr1(static_cast<uint32_t>(v >> 32), static_cast<uint32_t>(fill >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t wave_shl1_64(uint64_t v, uint64_t fill) {
+  const uint32_t lo = wave_shl1(static_cast<uint32_t>(v), static_cast<uint32_t>(fill));
+  const uint32_t hi = wave_shl1(static_cast<uint32_t>(v >> 32), static_cast<uint32_t>(fill >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 // Quad permutes through DPP: partner lane (lane ^ 1) / (lane ^ 2) within each
 // group of four lanes.
 __device__ __forceinline__ float quad_xor1(float v) {
@@ -103,11 +126,8 @@ struct WaveSorted {
 #pragma unroll
     for (int e = E - 1; e >= 0; --e) {
       const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
-      uint64_t up = from_lane64(v[e], lane == 0 ? 0 : lane - 1);
-      if (e > 0) {
-        const uint64_t carry = rdlane64(v[e - 1], 63);
-        if (lane == 0) up = carry;
-      }
+      // Lane 0 of row e takes the last element of row e - 1 (unused in row 0).
+      const uint64_t up = wave_shr1_64(v[e], e > 0 ? rdlane64(v[e > 0 ? e - 1 : 0], 63) : kKeyInf);
       uint64_t nv = v[e];
       if (k == pos) nv = x;
       if (k > pos) nv = up;
@@ -132,17 +152,10 @@ struct WaveSorted {
 #pragma unroll
     for (int e = E - 1; e >= 0; --e) {
       const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
-      const int src = lane == 0 ? 0 : lane - 1;
-      uint64_t up = from_lane64(v[e], src);
-      uint32_t fup = from_lane(fl[e], src);
-      if (e > 0) {
-        const uint64_t carry = rdlane64(v[e - 1], 63);
-        const uint32_t fcarry = rdlane(fl[e - 1], 63);
-        if (lane == 0) {
-          up = carry;
-          fup = fcarry;
-        }
-      }
+      // Lane 0 of row e takes the last entry of row e - 1 (unused in row 0).
+      const int pe = e > 0 ? e - 1 : 0;
+      const uint64_t up = wave_shr1_64(v[e], e > 0 ? rdlane64(v[pe], 63) : kKeyInf);
+      const uint32_t fup = wave_shr1(fl[e], e > 0 ? rdlane(fl[pe], 63) : 0u);
       uint64_t nv = v[e];
       uint32_t nf = fl[e];
       if (k == pos) {
@@ -164,14 +177,11 @@ struct WaveSorted {
   }
   /// Remove element 0.
   __device__ __forceinline__ void pop_front() {
-    const int lane = lane_id();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      uint64_t dn = from_lane64(v[e], lane == 63 ? 63 : lane + 1);
-      uint64_t carry = kKeyInf;
-      if (e + 1 < E) carry = rdlane64(v[e + 1 < E ? e + 1 : e], 0);
-      if (lane == 63) dn = carry;
-      v[e] = dn;
+      // Lane 63 of row e takes the first element of row e + 1 (empty after the last row).
+      const uint64_t carry = e + 1 < E ? rdlane64(v[e + 1 < E ? e + 1 : e], 0) : kKeyInf;
+      v[e] = wave_shl1_64(v[e], carry);
     }
   }
 };

@@ -121,13 +121,28 @@ __device__ __forceinline__ void row_dists(const DevIndexView& ix, const WaveLds&
   }
 }
 
+template <int ctrl, int row_mask>
+__device__ __forceinline__ uint64_t dpp_min64(uint64_t v) {
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+      static_cast<int>(static_cast<uint32_t>(v)), static_cast<int>(static_cast<uint32_t>(v)), ctrl, row_mask, 0xF, false));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+      static_cast<int>(static_cast<uint32_t>(v >> 32)), static_cast<int>(static_cast<uint32_t>(v >> 32)), ctrl,
+      row_mask, 0xF, false));
+  const uint64_t o = (static_cast<uint64_t>(hi) << 32) | lo;
+  return o < v ? o : v;
+}
+
+/// Minimum over the wave (all 64 lanes active), by DPP: quads, half rows and
+/// rows of 16 by lane permutes, then rows 1 and 3 take rows 0 and 2 (row_bcast
+/// of lane 15) and rows 2-3 take lane 31: lane 63 holds the minimum.
 __device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
-#pragma unroll
-  for (int s = 1; s < 64; s <<= 1) {
-    const uint64_t o = from_lane64(v, lane_id() ^ s);
-    v = o < v ? o : v;
-  }
-  return uniform64(v);
+  v = dpp_min64<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v = dpp_min64<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v = dpp_min64<0x141, 0xF>(v);  // row_half_mirror
+  v = dpp_min64<0x140, 0xF>(v);  // row_mirror
+  v = dpp_min64<0x142, 0xA>(v);  // row_bcast:15
+  v = dpp_min64<0x143, 0xC>(v);  // row_bcast:31
+  return rdlane64(v, 63);
 }
 
 /// Insert id into the visited hash; true iff it was not there.
@@ -154,11 +169,16 @@ __device__ __forceinline__ bool hash_has(const uint32_t* tab, uint32_t log2h, ui
   }
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 /// Compact LDS hash: 2^L slots of 16 bits, two per LDS word (0xFFFF empty).
 /// Ids are permuted by an odd multiplier mod 2^B (a bijection on B-bit ids);
 /// the top L bits of the permuted id give the home slot and the entry stores
 /// the other B - L bits with the distance from home, so an entry identifies
-/// its id exactly (linear probing, at most `maxp` probes).
+/// its id exactly (linear probing, at most `maxp` probes). Probes read the
+/// aligned group of 8 slots (16 bytes) holding the current slot at once and
+/// scan it in registers: most lookups cost one LDS read (and one
+/// compare-and-swap to insert). The table must be 16-byte aligned, L >= 3.
 struct Hash16 {
   uint32_t L, B, maxp;
 
@@ -167,39 +187,70 @@ struct Hash16 {
     home = x >> (B - L);
     rem = x & ((1u << (B - L)) - 1u);
   }
+  /// First slot j >= o of the group whose entry is `base + j * step` (the id
+  /// at that probe distance) or empty; 8 if none. maxj bounds j (probe limit).
+  __device__ __forceinline__ uint32_t scan(const u32x4 g, uint32_t o, uint32_t base, uint32_t step,
+                                           uint32_t maxj, bool& hit) const {
+    const uint32_t wd[4] = {g.x, g.y, g.z, g.w};
+    uint32_t stop = 8u;
+    bool h = false;
+#pragma unroll
+    for (int j = 7; j >= 0; --j) {
+      const uint32_t e = (wd[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
+      const bool match = e == base + static_cast<uint32_t>(j) * step;
+      if (static_cast<uint32_t>(j) >= o && static_cast<uint32_t>(j) <= maxj && (match || e == 0xFFFFu)) {
+        stop = static_cast<uint32_t>(j);
+        h = match;
+      }
+    }
+    hit = h;
+    return stop;
+  }
   /// 1: inserted, 0: present, 2: no free slot within maxp probes.
   __device__ __forceinline__ int visit(uint32_t* words, uint32_t id) const {
     uint32_t home, rem;
     place(id, home, rem);
-    const uint32_t smask = (1u << L) - 1u;
-    for (uint32_t p = 0; p <= maxp; ++p) {
+    const uint32_t smask = (1u << L) - 1u, S = B - L, step = 1u << S;
+    uint32_t p = 0;  // probe distance of the first slot to examine
+    for (;;) {
       const uint32_t slot = (home + p) & smask;
-      const uint32_t want = (p << (B - L)) | rem;
-      uint32_t* wp = words + (slot >> 1);
-      const uint32_t sh = (slot & 1u) << 4;
-      uint32_t cur = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      for (;;) {
-        const uint32_t e = (cur >> sh) & 0xFFFFu;
-        if (e == want) return 0;
-        if (e != 0xFFFFu) break;
-        const uint32_t old = atomicCAS(wp, cur, (cur & ~(0xFFFFu << sh)) | (want << sh));
+      const uint32_t o = slot & 7u, gi = slot >> 3;
+      const u32x4 g = *reinterpret_cast<const volatile u32x4*>(words + 4 * gi);
+      // Entry of slot j of this group: ((p - o + j) << S) | rem, i.e. base + j * step.
+      const uint32_t base = ((p - o) << S) + rem;
+      const uint32_t maxj = maxp - p + o;  // p - o + j <= maxp
+      bool hit;
+      const uint32_t j = scan(g, o, base, step, maxj, hit);
+      if (j < 8u) {
+        if (hit) return 0;
+        const uint32_t wi = j >> 1, sh = (j & 1u) * 16u;
+        const uint32_t cur = wi == 0 ? g.x : (wi == 1 ? g.y : (wi == 2 ? g.z : g.w));
+        const uint32_t want = base + j * step;
+        const uint32_t old = atomicCAS(words + 4 * gi + wi, cur, (cur & ~(0xFFFFu << sh)) | (want << sh));
         if (old == cur) return 1;
-        cur = old;
+        p = p - o + j;  // the word changed under us: look at this slot again
+        continue;
       }
+      p += 8u - o;
+      if (p > maxp) return 2;
     }
-    return 2;
   }
   __device__ __forceinline__ bool has(const uint32_t* words, uint32_t id) const {
     uint32_t home, rem;
     place(id, home, rem);
-    const uint32_t smask = (1u << L) - 1u;
-    for (uint32_t p = 0; p <= maxp; ++p) {
+    const uint32_t smask = (1u << L) - 1u, S = B - L, step = 1u << S;
+    uint32_t p = 0;
+    for (;;) {
       const uint32_t slot = (home + p) & smask;
-      const uint32_t e = (words[slot >> 1] >> ((slot & 1u) << 4)) & 0xFFFFu;
-      if (e == ((p << (B - L)) | rem)) return true;
-      if (e == 0xFFFFu) return false;
+      const uint32_t o = slot & 7u, gi = slot >> 3;
+      const u32x4 g = *reinterpret_cast<const volatile u32x4*>(words + 4 * gi);
+      const uint32_t base = ((p - o) << S) + rem;
+      bool hit;
+      const uint32_t j = scan(g, o, base, step, maxp - p + o, hit);
+      if (j < 8u) return hit;
+      p += 8u - o;
+      if (p > maxp) return false;
     }
-    return false;
   }
 };
 
@@ -559,7 +610,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
       }
     } else if (n_ties > 0) {
       c = key_of(tau, rdlane(ties, 0));
-      ties = from_lane(ties, lane == 63 ? 63 : lane + 1);
+      ties = wave_shl1(ties, rdlane(ties, 63));
       --n_ties;
     } else {
       break;
@@ -571,7 +622,8 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
     const uint32_t cid = key_id(c);
     const ListRegs cur = uniform(next.id) == cid ? next : load_list0(ix, cid);
     const uint32_t peek = first_unexpanded<ER>(fl, sR);
-    if (peek != ~0u) next = load_list0(ix, key_id(R.at(peek)));
+    const uint64_t peek_key = peek != ~0u ? R.at(peek) : kKeyInf;
+    if (peek != ~0u) next = load_list0(ix, key_id(peek_key));
 
     const uint32_t U = fresh_from_list(ix, w, cur, nullptr, [&](uint32_t id) { return vis.visit(id); });
     if (vis.compact && !vis.settle()) {
@@ -585,7 +637,8 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
     wave_sync();
     const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
     const uint32_t id_u = w.nid[lane];
-    uint64_t cand = ballot(static_cast<uint32_t>(lane) < U && (sR < ef || od_u < tau));
+    const bool pass = static_cast<uint32_t>(lane) < U && (sR < ef || od_u < tau);
+    uint64_t cand = ballot(pass);
     while (cand != 0) {
       const int i = __builtin_ctzll(cand);
       cand &= cand - 1;
@@ -610,7 +663,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
         const uint32_t tid = key_id(out);
         const uint32_t pos =
             static_cast<uint32_t>(__popcll(ballot(static_cast<uint32_t>(lane) < n_ties && ties < tid)));
-        const uint32_t up = from_lane(ties, lane == 0 ? 0 : lane - 1);
+        const uint32_t up = wave_shr1(ties, rdlane(ties, 0));
         if (static_cast<uint32_t>(lane) > pos) ties = up;
         if (static_cast<uint32_t>(lane) == pos) ties = tid;
         ++n_ties;
@@ -885,7 +938,7 @@ __device__ Handoff run_query(const DevIndexView& ix, const QueryArgs& a, const S
 template <int ER, int EC, int PC, int FM, int ST>
 __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView ix, SearchRequest r,
                                                                    SearchScratch s) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __align__(16) uint32_t lds[];
   const int lane = lane_id();
   const uint32_t per_wave = lds_words_per_wave(r.hash_log2, ix.padded);
   uint32_t* base = lds + static_cast<size_t>(threadIdx.x >> 6) * per_wave;
@@ -960,7 +1013,7 @@ __device__ uint64_t heap_pop(uint64_t* h, uint32_t& n, bool max_heap) {
 
 __global__ void __launch_bounds__(kWavesPerBlock * 64) general_search(DevIndexView ix, SearchRequest r,
                                                                       SearchScratch s) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __align__(16) uint32_t lds[];
   const int lane = lane_id();
   const uint32_t wave_slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave_slot >= s.spill_waves) return;
