@@ -562,6 +562,95 @@ __device__ __forceinline__ uint32_t first_unexpanded(const uint32_t (&fl)[ER], u
   return slot;
 }
 
+/// Admission of a whole expansion at once while R is full (|R| = ef): R
+/// becomes the ef smallest keys of R and of the candidates (lanes of `smask`,
+/// keys `skey`, all below tau), flags moving with their keys (candidates
+/// unexpanded). That is the sequential admission's result unless a candidate,
+/// or a key pushed out of R, has the new worst distance: then the arrival order
+/// decides ties, nothing is changed and false is returned. On success tau is
+/// the new worst distance, which is below the old one (keys were pushed out),
+/// so the caller's list of ties is empty.
+template <int ER>
+__device__ bool admit_merged(WaveSorted<ER>& R, uint32_t (&fl)[ER], uint32_t ef, uint32_t& tau, uint64_t smask,
+                             uint64_t skey) {
+  const int lane = lane_id();
+  const bool in_s = ((smask >> lane) & 1u) != 0;
+  // Merged position of every candidate: keys below it in R and among the candidates.
+  uint32_t spos = 0;
+  for (uint64_t m = smask; m != 0; m &= m - 1) {
+    const int j = __builtin_ctzll(m);
+    const uint64_t sj = rdlane64(skey, j);
+    const uint32_t p = R.rank(sj) + static_cast<uint32_t>(__popcll(ballot(in_s && skey < sj)));
+    if (lane == j) spos = p;
+  }
+  // The key after the last one kept (merged position ef) must be beyond the new worst distance.
+  const uint64_t at_ef = ballot(in_s && spos == ef);
+  const uint32_t s_before_ef = static_cast<uint32_t>(__popcll(ballot(in_s && spos < ef)));
+  const uint64_t first_out = at_ef != 0 ? rdlane64(skey, __builtin_ctzll(at_ef)) : R.at(ef - s_before_ef);
+  // Output slot k: the candidate placed there, else R[k - (candidates placed before k)].
+  uint32_t before[ER];
+  uint64_t nv[ER];
+  bool from_s[ER];
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    before[e] = 0;
+    nv[e] = kKeyInf;
+    from_s[e] = false;
+  }
+  for (uint64_t m = smask; m != 0; m &= m - 1) {
+    const int j = __builtin_ctzll(m);
+    const uint32_t pj = rdlane(spos, j);
+    const uint64_t sj = rdlane64(skey, j);
+#pragma unroll
+    for (int e = 0; e < ER; ++e) {
+      const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
+      before[e] += pj < k ? 1u : 0u;
+      if (pj == k) {
+        from_s[e] = true;
+        nv[e] = sj;
+      }
+    }
+  }
+  uint32_t nf[ER];
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
+    // At most 64 candidates: the source is in row e or row e - 1.
+    const uint32_t src = from_s[e] ? k : k - before[e];
+    const int sl = static_cast<int>(src & 63u);
+    uint64_t rv = from_lane64(R.v[e], sl);
+    uint32_t rf = from_lane(fl[e], sl);
+    if (e > 0) {
+      const uint64_t pv = from_lane64(R.v[e > 0 ? e - 1 : 0], sl);
+      const uint32_t pf = from_lane(fl[e > 0 ? e - 1 : 0], sl);
+      if ((src >> 6) != static_cast<uint32_t>(e)) {
+        rv = pv;
+        rf = pf;
+      }
+    }
+    nf[e] = from_s[e] ? 0u : rf;
+    if (!from_s[e]) nv[e] = rv;
+    if (k >= ef) {
+      nv[e] = kKeyInf;
+      nf[e] = 0u;
+    }
+  }
+  uint64_t worst = kKeyInf;
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    if (static_cast<uint32_t>(e) == ((ef - 1) >> 6)) worst = rdlane64(nv[e], static_cast<int>((ef - 1) & 63));
+  }
+  const uint32_t t = key_ord(worst);
+  if (key_ord(first_out) <= t || ballot(in_s && key_ord(skey) == t) != 0) return false;
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    R.v[e] = nv[e];
+    fl[e] = nf[e];
+  }
+  tau = t;
+  return true;
+}
+
 /// Layer-0 search without filter or deletions. Here every admitted node joins
 /// both C and R, so C needs no array of its own: its live entries are the
 /// entries of R not yet expanded (a flag per slot) plus the entries pushed out
@@ -637,8 +726,16 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
     wave_sync();
     const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
     const uint32_t id_u = w.nid[lane];
-    const bool pass = static_cast<uint32_t>(lane) < U && (sR < ef || od_u < tau);
-    uint64_t cand = ballot(pass);
+    uint64_t cand = ballot(static_cast<uint32_t>(lane) < U && (sR < ef || od_u < tau));
+    // Two or more candidates into a full R: one merge, unless ties at the new
+    // worst distance need the sequential order. (At ef <= 64 the merge's
+    // registers would cost a wavefront per SIMD: the sequential loop is kept.)
+    if constexpr (ER >= 2) {
+      if (sR == ef && (cand & (cand - 1)) != 0 && admit_merged<ER>(R, fl, ef, tau, cand, key_of(od_u, id_u))) {
+        n_ties = 0;
+        cand = 0;
+      }
+    }
     while (cand != 0) {
       const int i = __builtin_ctzll(cand);
       cand &= cand - 1;
