@@ -85,10 +85,10 @@ __device__ __forceinline__ uint64_t wave_shl1_64(uint64_t v, uint64_t fill) {
 // Quad permutes through DPP: partner lane (lane ^ 1) / (lane ^ 2) within each
 // group of four lanes.
 __device__ __forceinline__ float quad_xor1(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float quad_xor2(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
 }
 
 /// A sorted ascending array of u64 keys spread over the wave: element k lives in

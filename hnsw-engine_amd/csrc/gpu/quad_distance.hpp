@@ -85,7 +85,8 @@ __device__ __forceinline__ float quad_raw(bool l2, const float* __restrict__ row
 /// count pc is only known at run time), so a row costs one memory round trip.
 /// The summation order is quad_raw's: chunk c of a full 64-float block goes to
 /// accumulator c mod 4, later chunks to accumulator 0, each accumulator in
-/// increasing chunk order. Inactive quads return 0.
+/// increasing chunk order. Inactive quads return 0; their `row` must still be
+/// a readable row (it is loaded).
 template <int PC, bool L2>
 __device__ __forceinline__ float quad_raw_rows(const float* __restrict__ row, const float* q, uint32_t pc_rt,
                                                bool active) {
@@ -102,12 +103,8 @@ __device__ __forceinline__ float quad_raw_rows(const float* __restrict__ row, co
   for (uint32_t c0 = 0; c0 < pc; c0 += kGroup) {
     float4 x[kGroup];
 #pragma unroll
-    for (int j = 0; j < kGroup; ++j) x[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (active) {
-#pragma unroll
-      for (int j = 0; j < kGroup; ++j) {
-        if (PC > 0 || c0 + j < pc) x[j] = *reinterpret_cast<const float4*>(row + 16 * (c0 + j) + g4);
-      }
+    for (int j = 0; j < kGroup; ++j) {
+      if (PC > 0 || c0 + j < pc) x[j] = *reinterpret_cast<const float4*>(row + 16 * (c0 + j) + g4);
     }
 #pragma unroll
     for (int j = 0; j < kGroup; ++j) {

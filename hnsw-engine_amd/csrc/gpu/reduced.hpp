@@ -40,9 +40,10 @@ __device__ __forceinline__ float dot2_bf16(uint32_t a, uint32_t b, float c) {
 /// Dot product of a bf16 row with the bf16 query (both `padded` elements),
 /// by the four lanes of a quad: lane g takes 16-byte pieces g, g+4, ...
 /// (accumulator 0 the even element pairs of a piece, 1 the odd ones, pieces
-/// in increasing order). Identical in all four lanes. With PC (= padded / 16)
-/// known at compile time every load of the row is issued before the
-/// arithmetic.
+/// in increasing order). Identical in all four lanes; 0 for inactive quads.
+/// With PC (= padded / 16) known at compile time every load of the row is
+/// issued before the arithmetic, and `row` must be readable even when
+/// inactive.
 template <int PC>
 __device__ __forceinline__ float quad_dot_bf16(const uint16_t* __restrict__ row, const uint16_t* q,
                                                uint32_t padded, bool active) {
@@ -55,11 +56,7 @@ __device__ __forceinline__ float quad_dot_bf16(const uint16_t* __restrict__ row,
     constexpr int kPer = PC / 2;  // 2 PC pieces of 8 elements over 4 lanes
     uint4 x[kPer];
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) x[i] = make_uint4(0u, 0u, 0u, 0u);
-    if (active) {
-#pragma unroll
-      for (int i = 0; i < kPer; ++i) x[i] = r4[g + 4 * i];
-    }
+    for (int i = 0; i < kPer; ++i) x[i] = r4[g + 4 * i];
 #pragma unroll
     for (int i = 0; i < kPer; ++i) {
       const uint4 y = q4[g + 4 * i];
@@ -83,7 +80,7 @@ __device__ __forceinline__ float quad_dot_bf16(const uint16_t* __restrict__ row,
   float v = a0 + a1;
   v = v + quad_xor1(v);
   v = v + quad_xor2(v);
-  return v;
+  return active ? v : 0.0f;
 }
 
 __device__ __forceinline__ void fp8_piece(uint4 x, const float* q16, float& a0, float& a1) {
@@ -102,9 +99,9 @@ __device__ __forceinline__ void fp8_piece(uint4 x, const float* q16, float& a0, 
 }
 
 /// Dot product of an e4m3 row (before its scale) with the fp32 query: lane g
-/// of the quad takes 16-element pieces g, g+4, ... With PC (= padded / 16)
-/// known at compile time every load of the row is issued before the
-/// arithmetic.
+/// of the quad takes 16-element pieces g, g+4, ...; 0 for inactive quads. With
+/// PC (= padded / 16) known at compile time every load of the row is issued
+/// before the arithmetic, and `row` must be readable even when inactive.
 template <int PC>
 __device__ __forceinline__ float quad_dot_fp8(const uint8_t* __restrict__ row, const float* q, uint32_t padded,
                                               bool active) {
@@ -116,11 +113,7 @@ __device__ __forceinline__ float quad_dot_fp8(const uint8_t* __restrict__ row, c
     constexpr int kPer = PC / 4;  // PC pieces of 16 elements over 4 lanes
     uint4 x[kPer];
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) x[i] = make_uint4(0u, 0u, 0u, 0u);
-    if (active) {
-#pragma unroll
-      for (int i = 0; i < kPer; ++i) x[i] = r4[g + 4 * i];
-    }
+    for (int i = 0; i < kPer; ++i) x[i] = r4[g + 4 * i];
 #pragma unroll
     for (int i = 0; i < kPer; ++i) fp8_piece(x[i], q + 16 * (g + 4 * i), a0, a1);
   } else if (active) {
@@ -131,7 +124,7 @@ __device__ __forceinline__ float quad_dot_fp8(const uint8_t* __restrict__ row, c
   float v = a0 + a1;
   v = v + quad_xor1(v);
   v = v + quad_xor2(v);
-  return v;
+  return active ? v : 0.0f;
 }
 
 }  // namespace gpu

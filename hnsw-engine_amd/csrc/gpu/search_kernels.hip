@@ -575,19 +575,9 @@ __device__ bool admit_merged(WaveSorted<ER>& R, uint32_t (&fl)[ER], uint32_t ef,
                              uint64_t skey) {
   const int lane = lane_id();
   const bool in_s = ((smask >> lane) & 1u) != 0;
-  // Merged position of every candidate: keys below it in R and among the candidates.
-  uint32_t spos = 0;
-  for (uint64_t m = smask; m != 0; m &= m - 1) {
-    const int j = __builtin_ctzll(m);
-    const uint64_t sj = rdlane64(skey, j);
-    const uint32_t p = R.rank(sj) + static_cast<uint32_t>(__popcll(ballot(in_s && skey < sj)));
-    if (lane == j) spos = p;
-  }
-  // The key after the last one kept (merged position ef) must be beyond the new worst distance.
-  const uint64_t at_ef = ballot(in_s && spos == ef);
-  const uint32_t s_before_ef = static_cast<uint32_t>(__popcll(ballot(in_s && spos < ef)));
-  const uint64_t first_out = at_ef != 0 ? rdlane64(skey, __builtin_ctzll(at_ef)) : R.at(ef - s_before_ef);
-  // Output slot k: the candidate placed there, else R[k - (candidates placed before k)].
+  // Per candidate j (uniform in its iteration): merged position p_j = keys
+  // below it in R and among the candidates. Output slot k holds the candidate
+  // with p_j = k, else R[k - (candidates placed before k)].
   uint32_t before[ER];
   uint64_t nv[ER];
   bool from_s[ER];
@@ -597,10 +587,14 @@ __device__ bool admit_merged(WaveSorted<ER>& R, uint32_t (&fl)[ER], uint32_t ef,
     nv[e] = kKeyInf;
     from_s[e] = false;
   }
+  uint32_t s_before_ef = 0;
+  uint64_t s_at_ef = kKeyInf;
   for (uint64_t m = smask; m != 0; m &= m - 1) {
     const int j = __builtin_ctzll(m);
-    const uint32_t pj = rdlane(spos, j);
     const uint64_t sj = rdlane64(skey, j);
+    const uint32_t pj = R.rank(sj) + static_cast<uint32_t>(__popcll(ballot(in_s && skey < sj)));
+    s_before_ef += pj < ef ? 1u : 0u;
+    if (pj == ef) s_at_ef = sj;
 #pragma unroll
     for (int e = 0; e < ER; ++e) {
       const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
@@ -611,6 +605,8 @@ __device__ bool admit_merged(WaveSorted<ER>& R, uint32_t (&fl)[ER], uint32_t ef,
       }
     }
   }
+  // The key after the last one kept (merged position ef) must be beyond the new worst distance.
+  const uint64_t first_out = s_at_ef != kKeyInf ? s_at_ef : R.at(ef - s_before_ef);
   uint32_t nf[ER];
 #pragma unroll
   for (int e = 0; e < ER; ++e) {
