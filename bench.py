@@ -227,6 +227,129 @@ def cpu_baseline(index_path: str, Q: np.ndarray, truth: np.ndarray, k: int, efs,
             "points": [{"ef": e, "recall": round(r, 5), "qps": round(q, 1)} for e, r, q in points]}
 
 
+def stored_distance_error(dev, X: np.ndarray, Q: np.ndarray, storage: int, metric: str, rows: int = 2000,
+                          queries: int = 100) -> dict:
+    """Traversal distance of the stored copy (bf16 rows, or e4m3 codes x scale) against the FP64
+    distance of the original vectors, on random (query, row) pairs: relative error quantiles.
+    The stored values are read back from the device; the arithmetic is done in FP64, so this is
+    the error of the stored copy (fp32 accumulation adds ~1e-7 relative)."""
+    rng = np.random.default_rng(77)
+    ids = rng.choice(X.shape[0], rows, replace=False)
+    P = int(np.ceil(X.shape[1] / 16) * 16)
+    if storage == 1:
+        vals = np.stack([dev.get_row(int(i), 1, P)[0] for i in ids]).astype(np.uint32)
+        xs = (vals << 16).view(np.float32).astype(np.float64)[:, :X.shape[1]]
+    else:
+        tab = np.zeros(256)
+        for code in range(256):
+            s_, e, m = code >> 7, (code >> 3) & 0xF, code & 7
+            v = (m / 8.0) * 2.0 ** -6 if e == 0 else (1 + m / 8.0) * 2.0 ** (e - 7)
+            tab[code] = np.nan if (e == 0xF and m == 7) else (-v if s_ else v)
+        got = [dev.get_row(int(i), 2, P) for i in ids]
+        xs = np.stack([tab[r] * sc for r, sc, _ in got])[:, :X.shape[1]]
+    x64 = X[ids].astype(np.float64)
+    q64 = Q[:queries].astype(np.float64)
+    if metric == "cosine":
+        x64 = x64 / np.linalg.norm(x64, axis=1, keepdims=True)
+        q64 = q64 / np.linalg.norm(q64, axis=1, keepdims=True)
+        qr = q64
+        if storage == 1:  # the query is rounded to bf16 as well
+            u = q64.astype(np.float32).view(np.uint32).astype(np.uint64)
+            qr = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        exact = 1.0 - q64 @ x64.T
+        approx = 1.0 - qr @ xs.T
+    else:
+        exact = (q64 ** 2).sum(1)[:, None] + (x64 ** 2).sum(1)[None, :] - 2.0 * q64 @ x64.T
+        approx = (q64 ** 2).sum(1)[:, None] + (xs ** 2).sum(1)[None, :] - 2.0 * q64 @ xs.T
+    rel = np.abs(approx - exact) / np.maximum(np.abs(exact), 1e-12)
+    return {"pairs": int(rel.size), "median_rel": float(f"{np.median(rel):.3e}"),
+            "p99_rel": float(f"{np.quantile(rel, 0.99):.3e}"), "max_rel": float(f"{rel.max():.3e}")}
+
+
+def precision_workload(args) -> int:
+    """The reduced-precision configurations at their own sizes: 1M x 100 cosine with bf16 rows
+    (k 10 and 100) and 200k x 960 L2 with fp8 e4m3 rows + per-vector scale (k 10). Each sweeps ef
+    for the fp32 traversal and the reduced one (exact fp32 re-rank of the final candidates) and
+    reports QPS at recall 0.95 for both, the stored copy's distance error against FP64 and the
+    recall against GPU brute-force ground truth (FP64-checked on a sample)."""
+    import hnswgpu
+    import workloads
+
+    log(f"device {hnswgpu.device_info()}")
+    if args.workload == "cosine100":
+        n, nq, metric, storage, ks, efs = 1_000_000, 10_000, "cosine", 1, (10, 100), (32, 64, 128, 256, 512)
+        X, Q = workloads.cosine_100(n, seed=3), workloads.cosine_100(nq, seed=1003)
+        name = "1M x 100 cosine (1,000-centre mixture, log-normal norms), bf16 rows + fp32 re-rank"
+    else:
+        n, nq, metric, storage, ks, efs = 200_000, 5_000, "l2", 2, (10,), (64, 128, 256)
+        X, Q = workloads.sparse_960(n, seed=4), workloads.sparse_960(nq, seed=1004)
+        name = "200k x 960 L2 (|N(0,1)| x 0.1, 10 % of dims zeroed), fp8 e4m3 rows + scale + fp32 re-rank"
+    dim = X.shape[1]
+    threads = args.build_threads or len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    ix = hnswgpu.Index(dim, metric, M=16, ef_construction=200)
+    ix.add(X, num_threads=threads)
+    log(f"built {n} x {dim} {metric} on {threads} threads in {time.perf_counter() - t0:.1f}s (max level {ix.max_level})")
+    kmax = max(ks)
+    bf = hnswgpu.BruteForceIndex(dim, metric)
+    bf.add(X)
+    truth = bf.search_rows(Q, kmax)["label"].astype(np.int64)
+    del bf
+    agree, sample = 0, 16
+    x64 = X.astype(np.float64)
+    if metric == "cosine":
+        x64 /= np.linalg.norm(x64, axis=1, keepdims=True)
+    for i in range(sample):
+        q = Q[i].astype(np.float64)
+        d = (1.0 - x64 @ (q / np.linalg.norm(q))) if metric == "cosine" else ((x64 - q) ** 2).sum(1)
+        agree += int(np.array_equal(np.lexsort((np.arange(n), d))[:kmax], truth[i]))
+    del x64
+    log(f"ground truth: GPU brute force, FP64 top-{kmax} agreement on {sample} queries {agree / sample:.3f}")
+    dev = ix.device()
+    dev.autotune(Q, ks[0], [e for e in efs if e <= 512])
+    stream = hnswgpu.Stream()
+    d_q = hnswgpu.DeviceBuffer.of(Q)
+    d_out = hnswgpu.DeviceBuffer(nq * kmax * hnswgpu.NEIGHBOR.itemsize)
+    result = {"workload": name, "n": n, "dim": dim, "nq": nq, "metric": metric,
+              "storage": {1: "bf16", 2: "fp8_e4m3_scale"}[storage], "fp64_truth_agreement": agree / sample, "k": {}}
+    for k in ks:
+        host_rows = np.empty(nq * k, hnswgpu.NEIGHBOR)
+        per = {}
+        for st in (0, storage):
+            dev.set_option("storage", st)
+            pts = []
+            for ef in efs:
+                if ef < k:
+                    continue
+                for _ in range(args.warmup):
+                    dev.search_device(d_q.ptr, nq, k, ef, d_out.ptr, stream=stream.handle)
+                stream.synchronize()
+                start, end = hnswgpu.Event(), hnswgpu.Event()
+                start.record(stream)
+                for _ in range(args.steps):
+                    dev.search_device(d_q.ptr, nq, k, ef, d_out.ptr, stream=stream.handle)
+                end.record(stream)
+                stream.synchronize()
+                ms = start.elapsed_ms(end) / args.steps
+                rows = d_out.download(host_rows).reshape(nq, k)
+                rec = recall_at(rows["label"].astype(np.int64), truth, k)
+                pts.append((ef, rec, nq / (ms / 1e3)))
+                log(f"{args.workload} k={k} storage={st} ef={ef} recall={rec:.4f} qps={nq / (ms / 1e3):,.0f} step={ms:.3f}ms")
+            v = qps_at(pts)
+            per["fp32" if st == 0 else "reduced"] = {
+                "qps_at_recall_0.95": round(v, 1) if math.isfinite(v) else None,
+                "points": [{"ef": e, "recall": round(r, 5), "qps": round(q, 1)} for e, r, q in pts]}
+        dev.set_option("storage", 0)
+        a, b = per["fp32"]["qps_at_recall_0.95"], per["reduced"]["qps_at_recall_0.95"]
+        per["reduced_over_fp32"] = round(b / a, 3) if a and b else None
+        result["k"][str(k)] = per
+    dev.set_option("storage", storage)
+    result["stored_distance_error_vs_fp64"] = stored_distance_error(dev, X, Q, storage, metric)
+    dev.set_option("storage", 0)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,6 +375,9 @@ def main() -> int:
     ap.add_argument("--serving-seconds", type=float, default=5.0)
     ap.add_argument("--profile-brute", action="store_true",
                     help="profiling run of the brute force only (n x 128 data, nq queries, k), one line")
+    ap.add_argument("--workload", default="mixture1m", choices=["mixture1m", "cosine100", "sparse960"],
+                    help="mixture1m: the headline line; cosine100 / sparse960: the reduced-precision "
+                         "workloads (1M x 100 cosine with bf16, 200k x 960 L2 with fp8 e4m3), one line each")
     args = ap.parse_args()
     if args.calibrate_copy:
         import hnswgpu
@@ -263,6 +389,8 @@ def main() -> int:
             hnswgpu.copy_device(dst, src, nbytes)
         print(json.dumps({"copy_bytes": nbytes, "launches": 3}), flush=True)
         return 0
+    if args.workload != "mixture1m" and args.gpus == 1:
+        return precision_workload(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the index is replicated "
               "per GPU with no exchange, so N GPUs serve N independent query streams).", file=sys.stderr)

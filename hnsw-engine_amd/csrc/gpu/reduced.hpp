@@ -40,10 +40,10 @@ __device__ __forceinline__ float dot2_bf16(uint32_t a, uint32_t b, float c) {
 /// Dot product of a bf16 row with the bf16 query (both `padded` elements),
 /// by the four lanes of a quad: lane g takes 16-byte pieces g, g+4, ...
 /// (accumulator 0 the even element pairs of a piece, 1 the odd ones, pieces
-/// in increasing order). Identical in all four lanes; 0 for inactive quads.
-/// With PC (= padded / 16) known at compile time every load of the row is
-/// issued before the arithmetic, and `row` must be readable even when
-/// inactive.
+/// in increasing order). Identical in all four lanes; 0 for inactive quads,
+/// whose `row` must still be readable (it is loaded). With PC (= padded / 16)
+/// known at compile time every load of the row is issued before the
+/// arithmetic.
 template <int PC>
 __device__ __forceinline__ float quad_dot_bf16(const uint16_t* __restrict__ row, const uint16_t* q,
                                                uint32_t padded, bool active) {
@@ -65,16 +65,27 @@ __device__ __forceinline__ float quad_dot_bf16(const uint16_t* __restrict__ row,
       a0 = dot2_bf16(x[i].z, y.z, a0);
       a1 = dot2_bf16(x[i].w, y.w, a1);
     }
-  } else if (active) {
+  } else {
+    // Groups of up to 4 pieces per lane (padded <= 128: one group), all loads
+    // of a group issued before its arithmetic.
     const uint32_t pieces = padded >> 3;
-#pragma unroll 4
-    for (uint32_t i = g; i < pieces; i += 4) {
-      const uint4 x = r4[i];
-      const uint4 y = q4[i];
-      a0 = dot2_bf16(x.x, y.x, a0);
-      a1 = dot2_bf16(x.y, y.y, a1);
-      a0 = dot2_bf16(x.z, y.z, a0);
-      a1 = dot2_bf16(x.w, y.w, a1);
+    for (uint32_t i0 = 0; 4 * i0 < pieces; i0 += 4) {
+      uint4 x[4];
+#pragma unroll
+      for (uint32_t i = 0; i < 4; ++i) {
+        const uint32_t pi = g + 4 * (i0 + i);
+        if (pi < pieces) x[i] = r4[pi];
+      }
+#pragma unroll
+      for (uint32_t i = 0; i < 4; ++i) {
+        const uint32_t pi = g + 4 * (i0 + i);
+        if (pi >= pieces) break;
+        const uint4 y = q4[pi];
+        a0 = dot2_bf16(x[i].x, y.x, a0);
+        a1 = dot2_bf16(x[i].y, y.y, a1);
+        a0 = dot2_bf16(x[i].z, y.z, a0);
+        a1 = dot2_bf16(x[i].w, y.w, a1);
+      }
     }
   }
   float v = a0 + a1;
@@ -99,9 +110,10 @@ __device__ __forceinline__ void fp8_piece(uint4 x, const float* q16, float& a0, 
 }
 
 /// Dot product of an e4m3 row (before its scale) with the fp32 query: lane g
-/// of the quad takes 16-element pieces g, g+4, ...; 0 for inactive quads. With
-/// PC (= padded / 16) known at compile time every load of the row is issued
-/// before the arithmetic, and `row` must be readable even when inactive.
+/// of the quad takes 16-element pieces g, g+4, ...; 0 for inactive quads,
+/// whose `row` must still be readable (it is loaded). With PC (= padded / 16)
+/// known at compile time every load of the row is issued before the
+/// arithmetic.
 template <int PC>
 __device__ __forceinline__ float quad_dot_fp8(const uint8_t* __restrict__ row, const float* q, uint32_t padded,
                                               bool active) {
@@ -116,10 +128,24 @@ __device__ __forceinline__ float quad_dot_fp8(const uint8_t* __restrict__ row, c
     for (int i = 0; i < kPer; ++i) x[i] = r4[g + 4 * i];
 #pragma unroll
     for (int i = 0; i < kPer; ++i) fp8_piece(x[i], q + 16 * (g + 4 * i), a0, a1);
-  } else if (active) {
+  } else {
+    // Groups of up to 8 pieces per lane (padded <= 512: one group), all loads
+    // of a group issued before its arithmetic.
     const uint32_t pieces = padded >> 4;  // 16 elements per piece
-#pragma unroll 2
-    for (uint32_t i = g; i < pieces; i += 4) fp8_piece(r4[i], q + 16 * i, a0, a1);
+    for (uint32_t i0 = 0; 4 * i0 < pieces; i0 += 8) {
+      uint4 x[8];
+#pragma unroll
+      for (uint32_t i = 0; i < 8; ++i) {
+        const uint32_t pi = g + 4 * (i0 + i);
+        if (pi < pieces) x[i] = r4[pi];
+      }
+#pragma unroll
+      for (uint32_t i = 0; i < 8; ++i) {
+        const uint32_t pi = g + 4 * (i0 + i);
+        if (pi >= pieces) break;
+        fp8_piece(x[i], q + 16 * pi, a0, a1);
+      }
+    }
   }
   float v = a0 + a1;
   v = v + quad_xor1(v);

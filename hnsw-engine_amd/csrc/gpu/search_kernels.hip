@@ -399,12 +399,18 @@ __device__ uint64_t start_query(const DevIndexView& ix, const SearchScratch& s, 
   return descend<PC, ST>(ix, w, cur, work);
 }
 
-/// Re-rank R (sR approximate keys) with exact fp32 distances, in place.
-template <int ER, int PC>
-__device__ void rerank_exact(const DevIndexView& ix, const WaveLds& w, WaveSorted<ER>& R, uint32_t sR) {
+/// Re-rank R (sR keys with distances from a reduced copy) with exact fp32
+/// distances: R becomes the min(k, sR) exactly smallest keys, sorted, which is
+/// all the output takes. XE register rows hold them (64 * XE >= min(k, sR)).
+/// Each batch of 64 keys is scored at once and only keys below the current
+/// k-th are inserted.
+template <int XE, int ER, int PC>
+__device__ void rerank_top(const DevIndexView& ix, const WaveLds& w, WaveSorted<ER>& R, uint32_t sR, uint32_t k) {
   const int lane = lane_id();
-  WaveSorted<ER> X;
+  const uint32_t cap = k < sR ? k : sR;
+  WaveSorted<XE> X;
   X.clear();
+  uint64_t worst = kKeyInf;  // X's key at cap - 1, kKeyInf until X holds cap keys
 #pragma unroll
   for (int e = 0; e < ER; ++e) {
     const uint32_t base = static_cast<uint32_t>(e) * 64u;
@@ -415,9 +421,22 @@ __device__ void rerank_exact(const DevIndexView& ix, const WaveLds& w, WaveSorte
     wave_sync();
     row_dists<PC, kStoreF32>(ix, w, cnt);
     wave_sync();
-    for (uint32_t u = 0; u < cnt; ++u) X.insert(key_of(uniform(w.nd[u]), uniform(w.nid[u])), sR);
+    uint64_t m = ballot(static_cast<uint32_t>(lane) < cnt && key_of(w.nd[lane], w.nid[lane]) < worst);
+    for (; m != 0; m &= m - 1) {
+      const uint32_t i = static_cast<uint32_t>(__builtin_ctzll(m));
+      const uint64_t x = key_of(uniform(w.nd[i]), uniform(w.nid[i]));
+      if (x >= worst) continue;
+      X.insert(x, cap);
+      worst = X.at(cap - 1);
+    }
   }
-  R = X;
+#pragma unroll
+  for (int e = 0; e < ER; ++e) R.v[e] = e < XE ? X.v[e < XE ? e : 0] : kKeyInf;
+}
+
+template <int ER, int PC>
+__device__ void rerank_exact(const DevIndexView& ix, const WaveLds& w, WaveSorted<ER>& R, uint32_t sR, uint32_t k) {
+  rerank_top<ER, ER, PC>(ix, w, R, sR, k);
 }
 
 /// A node's layer-0 list in registers: lane j holds entry j + 1, `cnt` the
@@ -767,7 +786,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
   if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
   vis.finish();
   if (result != Handoff::kNone) return result;
-  if constexpr (ST != kStoreF32) rerank_exact<ER, PC>(ix, w, R, sR);
+  if constexpr (ST != kStoreF32) rerank_exact<ER, PC>(ix, w, R, sR, a.k);
   write_results<ER>(ix, a, R, sR, work);
   return Handoff::kNone;
 }
@@ -1000,7 +1019,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
   if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
   vis.finish();
   if (result != Handoff::kNone) return result;
-  if constexpr (ST != kStoreF32) rerank_exact<ER, PC>(ix, w, R, sR);
+  if constexpr (ST != kStoreF32) rerank_exact<ER, PC>(ix, w, R, sR, a.k);
   write_results<ER>(ix, a, R, sR, work);
   return Handoff::kNone;
 }

@@ -120,3 +120,33 @@ def test_reduced_cosine_and_filters(golden, lib):
         assert np.array_equal(got["label"], exact["label"]), storage
         assert np.array_equal(got["distance"].view(np.uint32), exact["distance"].view(np.uint32))
     dev.set_option("storage", 0)
+
+
+def test_high_dim_960_fp32_parity_and_fp8_rerank(lib, oracle, tmp_path):
+    """Padded dim 960 (the generic row loop, 60 chunks in groups of 8): the fp32
+    traversal equals the oracle bit for bit, and the fp8 e4m3 traversal returns
+    the exact fp32 distances of its ids with recall close to the fp32 one."""
+    import workloads
+
+    X = workloads.sparse_960(3000, seed=14)
+    Q = workloads.sparse_960(120, seed=1014)
+    ix = lib.Index(960, "l2", M=16, ef_construction=100)
+    ix.add(X, num_threads=0)
+    ix.save(tmp_path / "s960.idx")
+    orc = oracle.load(tmp_path / "s960.idx")
+    for ef in (10, 64, 200):
+        got = ix.search_rows(Q, 10, ef)
+        want = orc.search(Q, 10, ef, order="avx512")
+        assert np.array_equal(got["label"], want["label"]), ef
+        assert np.array_equal(got["distance"].view(np.uint32), want["distance"].view(np.uint32)), ef
+    exact = ix.search_rows(Q, 10, 64)
+    dev = ix.device()
+    dev.set_option("storage", 2)
+    low = ix.search_rows(Q, 10, 64)
+    dev.set_option("storage", 0)
+    lab = low["label"].astype(np.int64)
+    want = np.array([[oracle.l2(Q[i], X[j]) for j in lab[i]] for i in range(Q.shape[0])], np.float32)
+    assert np.array_equal(low["distance"].view(np.uint32), want.view(np.uint32))
+    d64 = ((Q[:, None, :].astype(np.float64) - X[None, :, :]) ** 2).sum(-1)
+    truth = np.argsort(d64, axis=1, kind="stable")[:, :10]
+    assert workloads.recall(low["label"], truth, 10) >= workloads.recall(exact["label"], truth, 10) - 0.03
