@@ -140,6 +140,14 @@ def serving_bench(ix, Q: np.ndarray, seconds: float) -> dict:
                 "graph_launches": int(st["graph_launches"])}
 
     srv = hnswgpu.Server(dev, slots=3, max_batch=4096, storages=(1,))
+    # Warm-up: every slot captures one graph per (storage, ef class) on first
+    # use. Three requests of a full launch each occupy the three slots at once,
+    # for every combination; then a random mix.
+    for st in (0, 1):
+        for ef in (32, 128, 256, 512):
+            ts = [srv.submit(Q[:4096], 10, ef, None, storage=st) for _ in range(3)]
+            for t in ts:
+                srv.wait(t)
     warm = make(50)
     for t in [srv.submit(q, k, ef, f, storage=st) for q, k, ef, f, st in warm]:
         srv.wait(t)
