@@ -35,6 +35,13 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "hnsw-engine_amd"))
+# The serving section keeps 3 launches in flight on 3 streams, beside the
+# index's stream, this script's timing stream and the default stream. HIP maps
+# streams onto a pool of hardware queues (GPU_MAX_HW_QUEUES) and serialises
+# launches of streams that share one; with a pool of 4 two slots can share.
+# Set before the HIP runtime starts (the first HIP call, after this line).
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 METRIC = "QPS at recall@10≥0.95 (1M×128 L2), GPU FP32 vs BF16+rerank vs CPU; p99 ms"
 HBM_PEAK_GBS = 8000.0

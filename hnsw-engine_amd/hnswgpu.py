@@ -17,6 +17,14 @@ from pathlib import Path
 
 import numpy as np
 
+# The serving loop keeps several launches in flight, one stream per slot. HIP
+# maps streams onto a pool of hardware queues (4 per process by default) and
+# launches of streams that share a queue run one after another, so with the
+# index's own stream and the default stream beside three slots, two slots can
+# end up serialised. Unless the caller chose a value, ask for 8 queues; the
+# HIP runtime reads this once, when it starts (the first HIP call).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 _HERE = Path(__file__).resolve().parent
 LIB_PATH = _HERE / "libhnswgpu.so"
 
