@@ -35,13 +35,14 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "hnsw-engine_amd"))
-# The serving section keeps 3 launches in flight on 3 streams, beside the
+# The serving section keeps 6 launches in flight on 6 streams, beside the
 # index's stream, this script's timing stream and the default stream. HIP maps
 # streams onto a pool of hardware queues (GPU_MAX_HW_QUEUES) and serialises
-# launches of streams that share one; with a pool of 4 two slots can share.
+# launches of streams that share one; with a pool of 4 slots would share.
 # Set before the HIP runtime starts (the first HIP call, after this line).
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+SERVING_SLOTS = 6
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 12:
+    os.environ["GPU_MAX_HW_QUEUES"] = "12"
 
 METRIC = "QPS at recall@10≥0.95 (1M×128 L2), GPU FP32 vs BF16+rerank vs CPU; p99 ms"
 HBM_PEAK_GBS = 8000.0
@@ -110,7 +111,7 @@ def measured_traffic(ef: int, args) -> dict:
         return {"traffic": None}
 
 
-def serving_bench(ix, Q: np.ndarray, seconds: float) -> dict:
+def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3) -> dict:
     """The serving loop under the mixed load of a production index: requests
     of log-uniform batch size 1..512, k in {1, 10, 100}, ef in {32..512},
     30 % with a 10 %-density BitsetFilter, half traversing the BF16 copy, 1 %
@@ -146,13 +147,13 @@ def serving_bench(ix, Q: np.ndarray, seconds: float) -> dict:
                 "launches": int(st["launches"]), "mean_queries_per_launch": round(st["mean_batch"], 1),
                 "graph_launches": int(st["graph_launches"])}
 
-    srv = hnswgpu.Server(dev, slots=3, max_batch=4096, storages=(1,))
+    srv = hnswgpu.Server(dev, slots=slots, max_batch=4096, storages=(1,))
     # Warm-up: every slot captures one graph per (storage, ef class) on first
-    # use. Three requests of a full launch each occupy the three slots at once,
+    # use. One request of a full launch per slot occupies every slot at once,
     # for every combination; then a random mix.
     for st in (0, 1):
         for ef in (32, 128, 256, 512):
-            ts = [srv.submit(Q[:4096], 10, ef, None, storage=st) for _ in range(3)]
+            ts = [srv.submit(Q[:4096], 10, ef, None, storage=st) for _ in range(slots)]
             for t in ts:
                 srv.wait(t)
     warm = make(50)
@@ -193,7 +194,7 @@ def serving_bench(ix, Q: np.ndarray, seconds: float) -> dict:
     log(f"serving open loop {opened}")
     srv.close()
     return {"mix": "batch 1..512 log-uniform, k {1,10,100}, ef {32..512}, 30% filtered (10% density), "
-                   "50% bf16 traversal, 1% deleted; 3 slots, max 4096 queries per launch, graphs",
+                   f"50% bf16 traversal, 1% deleted; {slots} slots, max 4096 queries per launch, graphs",
             "closed_loop": closed, "open_loop_poisson": opened}
 
 
@@ -566,7 +567,7 @@ def main() -> int:
         cpu = cpu_baseline(index_path, Q, truth, k, efs, args.cpu_budget_s)
 
     # Last: it soft-deletes 1 % of the base.
-    serving = None if args.no_serving else serving_bench(ix, Q, args.serving_seconds)
+    serving = None if args.no_serving else serving_bench(ix, Q, args.serving_seconds, SERVING_SLOTS)
 
     line = {
         "metric": METRIC,

@@ -21,9 +21,10 @@ import numpy as np
 # maps streams onto a pool of hardware queues (4 per process by default) and
 # launches of streams that share a queue run one after another, so with the
 # index's own stream and the default stream beside three slots, two slots can
-# end up serialised. Unless the caller chose a value, ask for 8 queues; the
-# HIP runtime reads this once, when it starts (the first HIP call).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# end up serialised. Unless the caller chose a value, ask for 12 queues (room
+# for 8 slots); the HIP runtime reads this once, when it starts (the first HIP
+# call).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "12")
 
 _HERE = Path(__file__).resolve().parent
 LIB_PATH = _HERE / "libhnswgpu.so"
