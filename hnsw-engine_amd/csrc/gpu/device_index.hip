@@ -177,6 +177,11 @@ void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint
     check(hnswamd::gpu::launch_eligibility(ix, static_cast<const uint32_t*>(h.deleted.p), d_fw, fsize, by_id, e, st),
           "eligibility kernel");
     r.elig = e;
+    // Deletions alone: the unfiltered traversal with a side list for the
+    // deleted nodes it admits, when they are few (1 in 16 at most) or when
+    // forced; the side list overflows (and hands queries on) as they grow.
+    const bool few = static_cast<uint64_t>(h.num_deleted) * 16u <= ix.count;
+    r.deleted_only = d_fw == nullptr && (h.deleted_path == 2 || (h.deleted_path == 1 && few)) ? 1 : 0;
   }
   check(hnswamd::gpu::launch_prepare(ix, r, s, st), "prepare kernel");
   hipEvent_t b = nullptr, e = nullptr;
@@ -591,6 +596,9 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
       h->hash_log2 = static_cast<uint32_t>(value);
     } else if (n == "force_spill") {
       h->force_spill = value != 0 ? 1 : 0;
+    } else if (n == "deleted_path") {
+      if (value < 0 || value > 2) throw std::invalid_argument("deleted_path must be 0, 1 or 2");
+      h->deleted_path = static_cast<int>(value);
     } else if (n == "storage") {
       // 0 = traverse the fp32 rows; 1 = a bf16 copy; 2 = an fp8 e4m3 copy with
       // per-vector scales. Reduced copies re-rank the final candidates in fp32.
@@ -633,6 +641,8 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
       *value = h->hash_log2;
     } else if (n == "force_spill") {
       *value = h->force_spill;
+    } else if (n == "deleted_path") {
+      *value = h->deleted_path;
     } else if (n == "spill_waves") {
       *value = h->spill_waves;
     } else if (n.rfind("last_", 0) == 0 && n != "last_kernel_ns" && h->main.counters.p != nullptr) {

@@ -135,6 +135,7 @@ struct hnswgpu_index {
   uint32_t waves_per_cu = 16;
   uint32_t hash_log2 = 0;
   int force_spill = 0;
+  int deleted_path = 1;
   int collect_stats = 0;
   int kernel_timing = 0;
 

@@ -675,7 +675,15 @@ __device__ bool admit_merged(WaveSorted<ER>& R, uint32_t (&fl)[ER], uint32_t ef,
 /// expanded again, so popping "first unexpanded slot of R, else the smallest
 /// tie" is the exact Alg. 2 order, and the loop ends exactly when Alg. 2 would
 /// stop (the smallest remaining candidate is further than the worst result).
-template <int ER, int PC, int ST>
+///
+/// DEL: the eligibility bitmap a.elig excludes a few nodes (soft deletions). An
+/// admitted ineligible node joins C only: it goes to a sorted side list D of up
+/// to 64 keys, and the next candidate is the smallest of R's first unexpanded
+/// slot, the smallest tie and D's head. An entry of D further than the worst
+/// result of a full R is dead (never expanded); D's head being dead ends the
+/// search, as Alg. 2's stop rule does. A live key pushed out of a full D hands
+/// the query to the general kernel.
+template <int ER, int PC, int ST, bool DEL = false>
 __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s,
                                const WaveLds& w, Visited& vis, uint32_t qi) {
   const int lane = lane_id();
@@ -693,20 +701,33 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
 #pragma unroll
   for (int e = 0; e < ER; ++e) fl[e] = 0u;
   uint32_t ties = 0u, n_ties = 0u;
+  WaveSorted<1> D;
+  D.clear();
   wave_sync();
   if (lane == 0) (void)vis.visit(key_id(ep));
   vis.in_lds = 1;
   wave_sync();
-  uint32_t dummy;
-  R.insert_flagged(ep, ef, fl, 0u, dummy);
-  uint32_t sR = 1, tau = key_ord(ep);
+  uint32_t sR = 0, tau = kOrdInf;
+  if (!DEL || elig_bit(a.elig, key_id(ep))) {
+    uint32_t dummy;
+    R.insert_flagged(ep, ef, fl, 0u, dummy);
+    sR = 1;
+    tau = key_ord(ep);
+  } else {
+    D.insert(ep, 64u);
+  }
   ListRegs next = load_list0(ix, key_id(ep));
 
   Handoff result = Handoff::kNone;
   for (;;) {
     uint64_t c;
     const uint32_t slot = first_unexpanded<ER>(fl, sR);
-    if (slot != ~0u) {
+    const uint64_t dmin = DEL ? rdlane64(D.v[0], 0) : kKeyInf;
+    if (DEL && dmin < (slot != ~0u ? R.at(slot) : n_ties > 0 ? key_of(tau, rdlane(ties, 0)) : kKeyInf)) {
+      c = dmin;
+      if (sR >= ef && key_ord(c) > tau) break;
+      D.pop_front();
+    } else if (slot != ~0u) {
       c = R.at(slot);
 #pragma unroll
       for (int e = 0; e < ER; ++e) {
@@ -726,10 +747,15 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
     const uint32_t cid = key_id(c);
     const ListRegs cur = uniform(next.id) == cid ? next : load_list0(ix, cid);
     const uint32_t peek = first_unexpanded<ER>(fl, sR);
-    const uint64_t peek_key = peek != ~0u ? R.at(peek) : kKeyInf;
-    if (peek != ~0u) next = load_list0(ix, key_id(peek_key));
+    uint64_t peek_key = peek != ~0u ? R.at(peek) : kKeyInf;
+    if constexpr (DEL) {
+      const uint64_t d0 = rdlane64(D.v[0], 0);
+      peek_key = d0 < peek_key ? d0 : peek_key;
+    }
+    if (peek_key != kKeyInf) next = load_list0(ix, key_id(peek_key));
 
-    const uint32_t U = fresh_from_list(ix, w, cur, nullptr, [&](uint32_t id) { return vis.visit(id); });
+    const uint32_t U =
+        fresh_from_list(ix, w, cur, DEL ? a.elig : nullptr, [&](uint32_t id) { return vis.visit(id); });
     if (vis.compact && !vis.settle()) {
       result = Handoff::kVisited;
       break;
@@ -741,12 +767,15 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
     wave_sync();
     const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
     const uint32_t id_u = w.nid[lane];
+    const uint32_t el_u = DEL ? w.nel[lane] : 1u;
     uint64_t cand = ballot(static_cast<uint32_t>(lane) < U && (sR < ef || od_u < tau));
     // Two or more candidates into a full R: one merge, unless ties at the new
     // worst distance need the sequential order. (At ef <= 64 the merge's
     // registers would cost a wavefront per SIMD: the sequential loop is kept.)
+    // With deletions only when every candidate is eligible.
     if constexpr (ER >= 2) {
-      if (sR == ef && (cand & (cand - 1)) != 0 && admit_merged<ER>(R, fl, ef, tau, cand, key_of(od_u, id_u))) {
+      if (sR == ef && (cand & (cand - 1)) != 0 && (!DEL || (cand & ~ballot(el_u != 0u)) == 0) &&
+          admit_merged<ER>(R, fl, ef, tau, cand, key_of(od_u, id_u))) {
         n_ties = 0;
         cand = 0;
       }
@@ -757,6 +786,16 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
       const uint32_t od = rdlane(od_u, i);
       if (sR >= ef && od >= tau) continue;
       const uint64_t key = key_of(od, rdlane(id_u, i));
+      if constexpr (DEL) {
+        if (rdlane(el_u, i) == 0u) {
+          const uint64_t dout = D.insert(key, 64u);
+          if (dout != kKeyInf && (sR < ef || key_ord(dout) <= tau)) {
+            result = Handoff::kCandidates;
+            break;
+          }
+          continue;
+        }
+      }
       uint32_t out_flag;
       const uint64_t out = R.insert_flagged(key, ef, fl, 0u, out_flag);
       if (sR < ef) {
@@ -1025,8 +1064,9 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
 }
 
 /// Filter modes of the fast kernel: every query unfiltered, every query
-/// filtered, or per query (mixed launches).
-enum FilterMode : int { kPlain = 0, kFiltered = 1, kPerQuery = 2 };
+/// filtered, per query (mixed launches), or every query with an eligibility
+/// bitmap that excludes only a few deleted nodes.
+enum FilterMode : int { kPlain = 0, kFiltered = 1, kPerQuery = 2, kDeleted = 3 };
 
 /// One query. In a mixed launch (kPerQuery) the kernel is built for the
 /// largest capacity class of the batch and a query with a smaller ef runs the
@@ -1041,6 +1081,8 @@ __device__ Handoff run_query(const DevIndexView& ix, const QueryArgs& a, const S
     return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
   } else if constexpr (FM == kFiltered) {
     return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi, runs);
+  } else if constexpr (FM == kDeleted) {
+    return plain_query<ER, PC, ST, true>(ix, a, s, w, vis, qi);
   } else {
     if (a.elig == nullptr) return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
     return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi, runs);
@@ -1326,6 +1368,7 @@ hipError_t pick_capacity(const DevIndexView& ix, const SearchRequest& r, SearchS
 template <int PC, int ST>
 hipError_t pick_filter(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
   if (r.tasks != nullptr) return pick_capacity<PC, kPerQuery, ST>(ix, r, s, st);
+  if (r.elig != nullptr && r.deleted_only) return pick_capacity<PC, kDeleted, ST>(ix, r, s, st);
   if (r.elig != nullptr) return pick_capacity<PC, kFiltered, ST>(ix, r, s, st);
   return pick_capacity<PC, kPlain, ST>(ix, r, s, st);
 }

@@ -209,6 +209,10 @@ int hnswgpu_autotune(hnswgpu_index* h, const float* queries, uint64_t nq, uint64
  *   "use_tuned"      1 = the autotuner's launch shapes (set by hnswgpu_autotune)
  *   "storage"        traversal copy, below
  *   "force_spill"    1 = every query through the general (HBM heap) kernel
+ *   "deleted_path"   searches whose only exclusions are soft deletions: 0 = the
+ *                    filtered kernel, 1 = the unfiltered kernel with a side list
+ *                    for deleted nodes when at most 1 node in 16 is deleted
+ *                    (default), 2 = that kernel at any deletion count
  *   "spill_waves"    wavefronts of the general kernel
  *   "collect_stats"  1 = record per-query work counts (hnswgpu_read_stats)
  *   "kernel_timing"  1 = bracket the fast kernel with events on its stream

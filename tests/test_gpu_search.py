@@ -209,3 +209,60 @@ def test_work_counters_equal_oracle_counters(golden, lib, oracle):
     dev.set_option("collect_stats", 0)
     _, want = orc.search(Q, 10, 64, order="avx512", counters=True)
     assert np.array_equal(got, want)
+
+
+def test_deletions_side_list_kernel(golden, lib, oracle):
+    """Soft deletions alone run the unfiltered traversal with a side list of
+    admitted deleted nodes ("deleted_path" 2 forces it at any deletion count,
+    0 uses the filtered kernel): both equal the reference's golden rows and
+    the oracle, and a side list that overflows hands its queries on exactly."""
+    for name in case_names(golden):
+        print("case", name, flush=True)
+        ix = lib.Index.load(GOLDEN / f"{name}_avx512.idx")
+        Q = golden[f"{name}/queries"]
+        for lab in golden[f"{name}/deleted"].tolist():
+            ix.mark_deleted(lab)
+        dev = ix.device()
+        for path in (2, 0, 1):
+            dev.set_option("deleted_path", path)
+            assert rows_equal(ix.search_rows(Q, 10, 64), golden[f"{name}/avx512/deleted/k10_ef64"]), (name, path)
+        path = GOLDEN / f"{name}_scalar.idx"
+        ix, orc = lib.Index.load(path), oracle.load(path)
+        for lab in golden[f"{name}/deleted"].tolist():
+            ix.mark_deleted(lab)
+            orc.mark_deleted(lab)
+        ix.device().set_option("deleted_path", 2)
+        for k, ef in ((1, 1), (10, 10), (10, 64), (7, 300)):
+            assert rows_equal(ix.search_rows(Q, k, ef), orc.search(Q, k, ef, order="avx512")), (name, k, ef)
+
+    path = GOLDEN / "l2_d32_scalar.idx"
+    ix, orc = lib.Index.load(path), oracle.load(path)
+    Q = golden["l2_d32/queries"]
+    for lab in range(0, len(ix), 2):
+        ix.mark_deleted(lab)
+        orc.mark_deleted(lab)
+    dev = ix.device()
+    dev.set_option("deleted_path", 2)
+    for ef in (64, 200, 500):
+        rows = ix.search_rows(Q, 10, ef)
+        handed = dev.get_option("last_handoff_candidates")
+        print("half deleted, ef", ef, "handed on", handed, flush=True)
+        assert rows_equal(rows, orc.search(Q, 10, ef, order="avx512")), ef
+    assert handed > 0
+
+    # Work counts and a reduced copy: the side-list kernel and the filtered
+    # kernel run the same traversal.
+    ix = lib.Index.load(GOLDEN / "l2_d32_scalar.idx")
+    for lab in golden["l2_d32/deleted"].tolist():
+        ix.mark_deleted(lab)
+    dev = ix.device()
+    dev.set_option("collect_stats", 1)
+    got = {}
+    for st in (0, 1):
+        dev.set_option("storage", st)
+        for path in (2, 0):
+            dev.set_option("deleted_path", path)
+            got[st, path] = (ix.search_rows(Q, 10, 100), dev.read_stats(Q.shape[0]).copy())
+        assert rows_equal(got[st, 2][0], got[st, 0][0]), st
+        assert np.array_equal(got[st, 2][1], got[st, 0][1]), st
+    dev.set_option("collect_stats", 0)
