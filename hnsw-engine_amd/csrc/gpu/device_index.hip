@@ -177,11 +177,8 @@ void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint
     check(hnswamd::gpu::launch_eligibility(ix, static_cast<const uint32_t*>(h.deleted.p), d_fw, fsize, by_id, e, st),
           "eligibility kernel");
     r.elig = e;
-    // Deletions alone: the unfiltered traversal with a side list for the
-    // deleted nodes it admits, when they are few (1 in 16 at most) or when
-    // forced; the side list overflows (and hands queries on) as they grow.
-    const bool few = static_cast<uint64_t>(h.num_deleted) * 16u <= ix.count;
-    r.deleted_only = d_fw == nullptr && (h.deleted_path == 2 || (h.deleted_path == 1 && few)) ? 1 : 0;
+    // Deletions alone: the unfiltered traversal with a side list for the deleted nodes it admits.
+    r.deleted_only = d_fw == nullptr ? static_cast<int>(hnswamd::dev::deleted_side_list(h)) : 0;
   }
   check(hnswamd::gpu::launch_prepare(ix, r, s, st), "prepare kernel");
   hipEvent_t b = nullptr, e = nullptr;
@@ -212,6 +209,13 @@ namespace dev {
 
 uint64_t effective_ef(const hnswgpu_index& h, uint64_t k, uint64_t ef) {
   return std::max<uint64_t>(ef == 0 ? h.ef_search : ef, k);
+}
+
+uint32_t deleted_side_list(const hnswgpu_index& h) {
+  // When deleted nodes are few (1 in 16 at most) or when forced: the side list
+  // overflows, and hands queries to the general kernel, as they grow.
+  const bool few = static_cast<uint64_t>(h.num_deleted) * 16u <= h.view.count;
+  return h.deleted_path == 2 || (h.deleted_path == 1 && few) ? 1u : 0u;
 }
 
 gpu::DevIndexView view_for(const hnswgpu_index& h, int storage) {
@@ -360,6 +364,10 @@ void launch_tasks(hnswgpu_index& h, Scratch& sc, const hnswgpu_task* tasks, uint
   r.nq = static_cast<uint32_t>(n);
   r.ef = static_cast<uint32_t>(std::min<uint64_t>(max_ef, 0xFFFFFFFFull));
   r.force_spill = h.force_spill;
+  // Deletions and no filter in the whole launch: the side-list kernel.
+  bool any_filter = false;
+  for (uint64_t f = 0; f < nf; ++f) any_filter = any_filter || used[f] != 0;
+  r.deleted_only = del_only != nullptr && !any_filter ? static_cast<int>(deleted_side_list(h)) : 0;
   plan_request(h, r, r.nq);
   gpu::SearchScratch s = bind_scratch(h, sc, r, r.nq, spill_waves, st);
   check(gpu::launch_prepare(ix, r, s, st), "prepare kernel");

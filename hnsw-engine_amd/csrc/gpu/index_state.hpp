@@ -172,6 +172,9 @@ gpu::SearchScratch bind_scratch(hnswgpu_index& h, Scratch& sc, const gpu::Search
 uint32_t ef_class_index(uint32_t ef);
 /// Effective ef of a request: max(ef or ef_search, k).
 uint64_t effective_ef(const hnswgpu_index& h, uint64_t k, uint64_t ef);
+/// 1 when searches whose only exclusions are soft deletions run the unfiltered
+/// traversal with a side list of deleted nodes (option deleted_path).
+uint32_t deleted_side_list(const hnswgpu_index& h);
 /// Mixed launch of n tasks on `st`: eligibility bitmaps for the filters the
 /// tasks use, the task array staged in pinned `stage` and copied to
 /// `d_tasks`, then prepare + fast + general kernels with scratch `sc`.

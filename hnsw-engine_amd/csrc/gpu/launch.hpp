@@ -82,7 +82,8 @@ struct SearchRequest {
   uint32_t k;
   uint32_t ef;               // already clamped: max(ef or ef_search, k)
   const uint32_t* elig;      // id bitmap, 1 = may be returned; null = unfiltered
-  int deleted_only;          // 1: elig excludes only a few deleted nodes (the kernel keeps them in a side list)
+  int deleted_only;          // 1: every elig (tasks' included) excludes only a few deleted nodes, and is
+                             // set: the unfiltered kernel keeps the deleted nodes it admits in a side list
   OutRow* out;               // nq x k (device)
   uint32_t* counts;          // nq result counts (device) or null
   uint32_t* stats;           // nq x 3 work counts {distances, layer-0 lists, upper lists} or null

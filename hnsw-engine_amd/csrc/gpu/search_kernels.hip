@@ -1065,23 +1065,23 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
 
 /// Filter modes of the fast kernel: every query unfiltered, every query
 /// filtered, per query (mixed launches), or every query with an eligibility
-/// bitmap that excludes only a few deleted nodes.
-enum FilterMode : int { kPlain = 0, kFiltered = 1, kPerQuery = 2, kDeleted = 3 };
+/// bitmap that excludes only a few deleted nodes (one launch, or a mixed one).
+enum FilterMode : int { kPlain = 0, kFiltered = 1, kPerQuery = 2, kDeleted = 3, kPerQueryDeleted = 4 };
 
-/// One query. In a mixed launch (kPerQuery) the kernel is built for the
-/// largest capacity class of the batch and a query with a smaller ef runs the
-/// code of its own class.
+/// One query. In a mixed launch (kPerQuery, kPerQueryDeleted) the kernel is
+/// built for the largest capacity class of the batch and a query with a
+/// smaller ef runs the code of its own class.
 template <int ER, int EC, int PC, int FM, int ST>
 __device__ Handoff run_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s, const WaveLds& w,
                              Visited& vis, uint32_t qi, uint64_t* runs) {
-  if constexpr (FM == kPerQuery && ER > 1) {
+  if constexpr ((FM == kPerQuery || FM == kPerQueryDeleted) && ER > 1) {
     if (a.ef <= 32u * ER) return run_query<ER / 2, EC / 2, PC, FM, ST>(ix, a, s, w, vis, qi, runs);
   }
   if constexpr (FM == kPlain) {
     return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
   } else if constexpr (FM == kFiltered) {
     return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi, runs);
-  } else if constexpr (FM == kDeleted) {
+  } else if constexpr (FM == kDeleted || FM == kPerQueryDeleted) {
     return plain_query<ER, PC, ST, true>(ix, a, s, w, vis, qi);
   } else {
     if (a.elig == nullptr) return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
@@ -1367,7 +1367,10 @@ hipError_t pick_capacity(const DevIndexView& ix, const SearchRequest& r, SearchS
 
 template <int PC, int ST>
 hipError_t pick_filter(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
-  if (r.tasks != nullptr) return pick_capacity<PC, kPerQuery, ST>(ix, r, s, st);
+  if (r.tasks != nullptr) {
+    return r.deleted_only ? pick_capacity<PC, kPerQueryDeleted, ST>(ix, r, s, st)
+                          : pick_capacity<PC, kPerQuery, ST>(ix, r, s, st);
+  }
   if (r.elig != nullptr && r.deleted_only) return pick_capacity<PC, kDeleted, ST>(ix, r, s, st);
   if (r.elig != nullptr) return pick_capacity<PC, kFiltered, ST>(ix, r, s, st);
   return pick_capacity<PC, kPlain, ST>(ix, r, s, st);

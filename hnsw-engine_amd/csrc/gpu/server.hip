@@ -32,6 +32,7 @@
 #include <memory>
 #include <mutex>
 #include <stdexcept>
+#include <tuple>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -84,7 +85,7 @@ struct Slot {
   hnswamd::dev::Scratch sc;
   hnswamd::dev::PinnedBuf stage, result;
   hnswamd::dev::DevBuf in, out, elig;
-  std::map<std::pair<int, uint32_t>, Graph> graphs;  // (storage, ef class)
+  std::map<std::tuple<int, uint32_t, int>, Graph> graphs;  // (storage, ef class, deletions-only kernel)
   std::vector<Piece> pieces;
   uint32_t nq = 0;
   uint64_t rows = 0;
@@ -229,6 +230,7 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
   std::unique_lock<std::mutex> ilk(h.mu);  // index state (deletions, views) stays put while we read it
   const hnswamd::gpu::DevIndexView ix = hnswamd::dev::view_for(h, storage);
   const bool deletions = h.num_deleted > 0;
+  const uint32_t side_list = hnswamd::dev::deleted_side_list(h);
   *reinterpret_cast<uint32_t*>(base) = slot.nq;
   size_t var = srv.var_off;
   for (uint32_t f = 0; f <= c.max_filters; ++f) descs[f] = FilterDesc{nullptr, 0, 0u, 0u};
@@ -295,6 +297,8 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
   r.nq = c.max_batch;  // the grid covers the capacity; the kernels read the count
   r.ef = cls;
   r.force_spill = h.force_spill;
+  // Deletions and no filter in the whole launch: the side-list kernel.
+  r.deleted_only = deletions && filters.empty() ? static_cast<int>(side_list) : 0;
   hnswamd::dev::plan_request(h, r, r.nq);
   hnswamd::gpu::SearchScratch s = hnswamd::dev::bind_scratch(h, slot.sc, r, r.nq, c.spill_waves, slot.stream);
   auto* d_descs = reinterpret_cast<const FilterDesc*>(dbase + srv.descs_off);
@@ -310,7 +314,7 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
     check(hnswamd::gpu::launch_search(ix, r, s, slot.stream), "search kernel");
     check(hipMemcpyAsync(slot.result.p, slot.out.p, out_n, hipMemcpyDeviceToHost, slot.stream), "hipMemcpy out");
   };
-  const auto key = std::make_pair(storage, cls);
+  const auto key = std::make_tuple(storage, cls, r.deleted_only);
   auto it = slot.graphs.find(key);
   if (c.use_graphs && it != slot.graphs.end()) {
     Graph& g = it->second;

@@ -132,12 +132,19 @@ def test_deletions_and_errors(lib):
     dev = ix.device()
     srv = lib.Server(dev, slots=2, max_batch=256, max_filter_labels=5_000)
     mask = np.random.default_rng(3).random(5_000) < 0.4
-    ts = [srv.submit(Q[i * 20:(i + 1) * 20], 10, 50, mask if i % 2 else None) for i in range(10)]
-    for i, t in enumerate(ts):
-        rows, counts = srv.wait(t)
-        want, wc = dev.search_rows(Q[i * 20:(i + 1) * 20], 10, 50, mask if i % 2 else None)
-        assert rows_equal(rows, want) and np.array_equal(counts, wc)
-        assert not np.isin(rows["label"][rows["label"] != lib.INVALID_LABEL].astype(np.int64) % 13, [0]).any()
+    # The filtered kernel's rows; then served through it (deleted_path 0) and
+    # through the side-list kernel for deletions alone (2).
+    dev.set_option("deleted_path", 0)
+    wants = [dev.search_rows(Q[i * 20:(i + 1) * 20], 10, 50, mask if i % 2 else None) for i in range(10)]
+    for path in (0, 2):
+        dev.set_option("deleted_path", path)
+        ts = [srv.submit(Q[i * 20:(i + 1) * 20], 10, 50, mask if i % 2 else None) for i in range(10)]
+        for i, t in enumerate(ts):
+            rows, counts = srv.wait(t)
+            want, wc = wants[i]
+            assert rows_equal(rows, want) and np.array_equal(counts, wc), (path, i)
+            assert not np.isin(rows["label"][rows["label"] != lib.INVALID_LABEL].astype(np.int64) % 13, [0]).any()
+    dev.set_option("deleted_path", 1)
     with pytest.raises(ValueError):
         srv.submit(Q[:2], 0, 10)
     with pytest.raises(ValueError):
