@@ -148,14 +148,17 @@ def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3) -> dict:
                 "graph_launches": int(st["graph_launches"])}
 
     srv = hnswgpu.Server(dev, slots=slots, max_batch=4096, storages=(1,))
-    # Warm-up: every slot captures one graph per (storage, ef class) on first
-    # use. One request of a full launch per slot occupies every slot at once,
-    # for every combination; then a random mix.
+    # Warm-up: every slot captures one graph per (storage, ef class, kernel)
+    # on first use, the kernel being the mixed one when a launch holds a
+    # filtered request and the deletions side-list one when it holds none.
+    # One request of a full launch per slot occupies every slot at once, for
+    # every combination; then a random mix.
     for st in (0, 1):
         for ef in (32, 128, 256, 512):
-            ts = [srv.submit(Q[:4096], 10, ef, None, storage=st) for _ in range(slots)]
-            for t in ts:
-                srv.wait(t)
+            for f in (None, fw):
+                ts = [srv.submit(Q[:4096], 10, ef, f, storage=st) for _ in range(slots)]
+                for t in ts:
+                    srv.wait(t)
     warm = make(50)
     for t in [srv.submit(q, k, ef, f, storage=st) for q, k, ef, f, st in warm]:
         srv.wait(t)
