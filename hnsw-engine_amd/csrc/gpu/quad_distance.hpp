@@ -86,11 +86,13 @@ __device__ __forceinline__ float quad_raw(bool l2, const float* __restrict__ row
 /// The summation order is quad_raw's: chunk c of a full 64-float block goes to
 /// accumulator c mod 4, later chunks to accumulator 0, each accumulator in
 /// increasing chunk order. Inactive quads return 0; their `row` must still be
-/// a readable row (it is loaded).
-template <int PC, bool L2>
+/// a readable row (it is loaded). G: the group size of the run-time loop
+/// (fewer registers, more round trips when smaller).
+template <int PC, bool L2, int G = 8>
 __device__ __forceinline__ float quad_raw_rows(const float* __restrict__ row, const float* q, uint32_t pc_rt,
                                                bool active) {
-  constexpr int kGroup = PC > 0 ? PC : 8;
+  static_assert(G % 4 == 0, "groups of whole 64-float blocks");
+  constexpr int kGroup = PC > 0 ? PC : G;
   const uint32_t pc = PC > 0 ? static_cast<uint32_t>(PC) : pc_rt;
   const uint32_t full = (pc >> 2) << 2;
   const uint32_t g4 = static_cast<uint32_t>(lane_id() & 3) * 4;

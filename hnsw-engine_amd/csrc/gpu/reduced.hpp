@@ -44,7 +44,7 @@ __device__ __forceinline__ float dot2_bf16(uint32_t a, uint32_t b, float c) {
 /// whose `row` must still be readable (it is loaded). With PC (= padded / 16)
 /// known at compile time every load of the row is issued before the
 /// arithmetic.
-template <int PC>
+template <int PC, int G = 4>
 __device__ __forceinline__ float quad_dot_bf16(const uint16_t* __restrict__ row, const uint16_t* q,
                                                uint32_t padded, bool active) {
   const uint32_t g = static_cast<uint32_t>(lane_id() & 3);
@@ -66,18 +66,18 @@ __device__ __forceinline__ float quad_dot_bf16(const uint16_t* __restrict__ row,
       a1 = dot2_bf16(x[i].w, y.w, a1);
     }
   } else {
-    // Groups of up to 4 pieces per lane (padded <= 128: one group), all loads
-    // of a group issued before its arithmetic.
+    // Groups of up to G pieces per lane (G = 4: padded <= 128 in one group),
+    // all loads of a group issued before its arithmetic.
     const uint32_t pieces = padded >> 3;
-    for (uint32_t i0 = 0; 4 * i0 < pieces; i0 += 4) {
-      uint4 x[4];
+    for (uint32_t i0 = 0; 4 * i0 < pieces; i0 += G) {
+      uint4 x[G];
 #pragma unroll
-      for (uint32_t i = 0; i < 4; ++i) {
+      for (uint32_t i = 0; i < G; ++i) {
         const uint32_t pi = g + 4 * (i0 + i);
         if (pi < pieces) x[i] = r4[pi];
       }
 #pragma unroll
-      for (uint32_t i = 0; i < 4; ++i) {
+      for (uint32_t i = 0; i < G; ++i) {
         const uint32_t pi = g + 4 * (i0 + i);
         if (pi >= pieces) break;
         const uint4 y = q4[pi];

@@ -82,7 +82,7 @@ __device__ __forceinline__ WaveLds wave_lds(uint32_t* base, uint32_t log2h, uint
 /// from the fp32 rows (ST = 0, exact) or the bf16 / fp8 copy (ST = 1 / 2).
 /// Every load a row needs (its norm and scale included) is issued before its
 /// arithmetic, so a step costs one memory round trip.
-template <int PC, int ST, bool L2>
+template <int PC, int ST, bool L2, int G, int BG>
 __device__ __forceinline__ void row_dists_metric(const DevIndexView& ix, const WaveLds& w, uint32_t U) {
   const int lane = lane_id();
   const float qn = ST == kStoreF32 ? 0.0f : w.qn[0];
@@ -93,14 +93,14 @@ __device__ __forceinline__ void row_dists_metric(const DevIndexView& ix, const W
     float d;
     if constexpr (ST == kStoreF32) {
       const float* row = ix.vecs + static_cast<size_t>(id) * ix.padded;
-      d = quad_raw_rows<PC, L2>(row, w.q, ix.padded >> 4, active);
+      d = quad_raw_rows<PC, L2, G>(row, w.q, ix.padded >> 4, active);
       if (!L2) d = 1.0f - d;
     } else {
       const float xn = (L2 && active) ? ix.vnorm[id] : 0.0f;
       float dp;
       if constexpr (ST == kStoreBF16) {
         const uint16_t* row = static_cast<const uint16_t*>(ix.vlow) + static_cast<size_t>(id) * ix.padded;
-        dp = quad_dot_bf16<PC>(row, w.qb, ix.padded, active);
+        dp = quad_dot_bf16<PC, BG>(row, w.qb, ix.padded, active);
       } else {
         const float xs = active ? ix.vscale[id] : 0.0f;
         const uint8_t* row = static_cast<const uint8_t*>(ix.vlow) + static_cast<size_t>(id) * ix.padded;
@@ -112,12 +112,15 @@ __device__ __forceinline__ void row_dists_metric(const DevIndexView& ix, const W
   }
 }
 
-template <int PC, int ST = kStoreF32>
+/// G / BG: group sizes of the run-time-width fp32 / bf16 row loops. The
+/// largest capacity class (ER = 8) takes smaller groups: its registers would
+/// otherwise spill to scratch.
+template <int PC, int ST = kStoreF32, int G = 8, int BG = 4>
 __device__ __forceinline__ void row_dists(const DevIndexView& ix, const WaveLds& w, uint32_t U) {
   if (ix.metric == 0) {
-    row_dists_metric<PC, ST, true>(ix, w, U);
+    row_dists_metric<PC, ST, true, G, BG>(ix, w, U);
   } else {
-    row_dists_metric<PC, ST, false>(ix, w, U);
+    row_dists_metric<PC, ST, false, G, BG>(ix, w, U);
   }
 }
 
@@ -334,7 +337,7 @@ struct Work {
 /// Greedy descent from `cur` through layers max_level..1. A list's count and
 /// entries are loaded together, and each neighbour's list offset is fetched
 /// with its row, so a step costs two memory round trips (list, then rows).
-template <int PC, int ST>
+template <int PC, int ST, int BG = 4>
 __device__ uint64_t descend(const DevIndexView& ix, const WaveLds& w, uint64_t cur, Work& work) {
   const int lane = lane_id();
   if (ix.max_level <= 0) return cur;
@@ -352,7 +355,7 @@ __device__ uint64_t descend(const DevIndexView& ix, const WaveLds& w, uint64_t c
       if (has) w.nid[lane] = nb;
       const uint32_t nb_off = has ? ix.upper_off[nb] : 0u;
       wave_sync();
-      row_dists<PC, ST>(ix, w, cnt);
+      row_dists<PC, ST, 8, BG>(ix, w, cnt);
       wave_sync();
       const uint64_t mine = has ? key_of(w.nd[lane], static_cast<uint32_t>(lane)) : kKeyInf;
       const uint64_t best = wave_min64(mine);
@@ -366,7 +369,7 @@ __device__ uint64_t descend(const DevIndexView& ix, const WaveLds& w, uint64_t c
 }
 
 /// Load the prepared query into LDS and compute the entry point distance.
-template <int PC, int ST = kStoreF32>
+template <int PC, int ST = kStoreF32, int BG = 4>
 __device__ uint64_t start_query(const DevIndexView& ix, const SearchScratch& s, const WaveLds& w, uint32_t qi,
                                 Work& work) {
   const int lane = lane_id();
@@ -392,11 +395,11 @@ __device__ uint64_t start_query(const DevIndexView& ix, const SearchScratch& s, 
   }
   if (lane == 0) w.nid[0] = ix.entry;
   wave_sync();
-  row_dists<PC, ST>(ix, w, 1);
+  row_dists<PC, ST, 8, BG>(ix, w, 1);
   wave_sync();
   const uint64_t cur = key_of(uniform(w.nd[0]), ix.entry);
   work.dist += 1;
-  return descend<PC, ST>(ix, w, cur, work);
+  return descend<PC, ST, BG>(ix, w, cur, work);
 }
 
 /// Re-rank R (sR keys with distances from a reduced copy) with exact fp32
@@ -404,7 +407,7 @@ __device__ uint64_t start_query(const DevIndexView& ix, const SearchScratch& s, 
 /// all the output takes. XE register rows hold them (64 * XE >= min(k, sR)).
 /// Each batch of 64 keys is scored at once and only keys below the current
 /// k-th are inserted.
-template <int XE, int ER, int PC>
+template <int XE, int ER, int PC, int G>
 __device__ void rerank_top(const DevIndexView& ix, const WaveLds& w, WaveSorted<ER>& R, uint32_t sR, uint32_t k) {
   const int lane = lane_id();
   const uint32_t cap = k < sR ? k : sR;
@@ -419,7 +422,7 @@ __device__ void rerank_top(const DevIndexView& ix, const WaveLds& w, WaveSorted<
     wave_sync();
     if (static_cast<uint32_t>(lane) < cnt) w.nid[lane] = key_id(R.v[e]);
     wave_sync();
-    row_dists<PC, kStoreF32>(ix, w, cnt);
+    row_dists<PC, kStoreF32, G>(ix, w, cnt);
     wave_sync();
     uint64_t m = ballot(static_cast<uint32_t>(lane) < cnt && key_of(w.nd[lane], w.nid[lane]) < worst);
     for (; m != 0; m &= m - 1) {
@@ -434,9 +437,9 @@ __device__ void rerank_top(const DevIndexView& ix, const WaveLds& w, WaveSorted<
   for (int e = 0; e < ER; ++e) R.v[e] = e < XE ? X.v[e < XE ? e : 0] : kKeyInf;
 }
 
-template <int ER, int PC>
+template <int ER, int PC, int G>
 __device__ void rerank_exact(const DevIndexView& ix, const WaveLds& w, WaveSorted<ER>& R, uint32_t sR, uint32_t k) {
-  rerank_top<ER, ER, PC>(ix, w, R, sR, k);
+  rerank_top<ER, ER, PC, G>(ix, w, R, sR, k);
 }
 
 /// A node's layer-0 list in registers: lane j holds entry j + 1, `cnt` the
@@ -693,7 +696,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
   vis.in_lds = 0;
   vis.in_hbm = 0;
   Work work;
-  const uint64_t ep = start_query<PC, ST>(ix, s, w, qi, work);
+  const uint64_t ep = start_query<PC, ST, (ER >= 8 ? 2 : 4)>(ix, s, w, qi, work);
 
   WaveSorted<ER> R;
   R.clear();
@@ -763,7 +766,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
     vis.count(U);
     work.dist += U;
     ++work.exp0;
-    row_dists<PC, ST>(ix, w, U);
+    row_dists<PC, ST, 8, (ER >= 8 ? 2 : 4)>(ix, w, U);
     wave_sync();
     const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
     const uint32_t id_u = w.nid[lane];
@@ -825,7 +828,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
   if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
   vis.finish();
   if (result != Handoff::kNone) return result;
-  if constexpr (ST != kStoreF32) rerank_exact<ER, PC>(ix, w, R, sR, a.k);
+  if constexpr (ST != kStoreF32) rerank_exact<ER, PC, (ER >= 8 && ST == kStoreBF16 ? 4 : 8)>(ix, w, R, sR, a.k);
   write_results<ER>(ix, a, R, sR, work);
   return Handoff::kNone;
 }
@@ -966,7 +969,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
   vis.in_lds = 0;
   vis.in_hbm = 0;
   Work work;
-  const uint64_t ep = start_query<PC, ST>(ix, s, w, qi, work);
+  const uint64_t ep = start_query<PC, ST, (ER >= 8 ? 2 : 4)>(ix, s, w, qi, work);
 
   WaveSorted<ER> R;
   WaveSorted<EC> C;
@@ -1022,7 +1025,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
     vis.count(U);
     work.dist += U;
     ++work.exp0;
-    row_dists<PC, ST>(ix, w, U);
+    row_dists<PC, ST, 8, (ER >= 8 ? 2 : 4)>(ix, w, U);
     wave_sync();
     const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
     const uint32_t id_u = w.nid[lane];
@@ -1058,7 +1061,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
   if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
   vis.finish();
   if (result != Handoff::kNone) return result;
-  if constexpr (ST != kStoreF32) rerank_exact<ER, PC>(ix, w, R, sR, a.k);
+  if constexpr (ST != kStoreF32) rerank_exact<ER, PC, (ER >= 8 && ST == kStoreBF16 ? 4 : 8)>(ix, w, R, sR, a.k);
   write_results<ER>(ix, a, R, sR, work);
   return Handoff::kNone;
 }

@@ -150,3 +150,40 @@ def test_high_dim_960_fp32_parity_and_fp8_rerank(lib, oracle, tmp_path):
     d64 = ((Q[:, None, :].astype(np.float64) - X[None, :, :]) ** 2).sum(-1)
     truth = np.argsort(d64, axis=1, kind="stable")[:, :10]
     assert workloads.recall(low["label"], truth, 10) >= workloads.recall(exact["label"], truth, 10) - 0.03
+
+
+def test_reduced_largest_class_generic_width(golden, lib):
+    """ef above 256 (the largest capacity class) at a dimension of no fixed
+    row width (100, padded to 112): plain, filtered and deletions-only
+    searches on the bf16 and fp8 copies return exact fp32 distances, sorted,
+    and the answer set of the fp32 search on 800 nodes (k 10 and 100)."""
+    ix = lib.Index.load(GOLDEN / "cos_d100_avx512.idx")
+    Q = golden["cos_d100/queries"]
+    filt = (golden["cos_d100/allow_words"], int(golden["cos_d100/allow_size"][0]))
+    dev = ix.device()
+    for deleted in (False, True):
+        if deleted:
+            for lab in golden["cos_d100/deleted"].tolist():
+                ix.mark_deleted(lab)
+        for f in (None, filt):
+            for k in (10, 100):
+                dev.set_option("storage", 0)
+                exact = ix.search_rows(Q, k, 400, filter=f)
+                for storage in (1, 2):
+                    dev.set_option("storage", storage)
+                    got = ix.search_rows(Q, k, 400, filter=f)
+                    case = (deleted, f is not None, k, storage)
+                    print("case", case, flush=True)
+                    hit = 0
+                    for i in range(Q.shape[0]):
+                        want = dict(zip(exact["label"][i].tolist(), exact["distance"][i].view(np.uint32).tolist()))
+                        n = int((got["label"][i] != lib.INVALID_LABEL).sum())
+                        assert np.all(got["label"][i][n:] == lib.INVALID_LABEL), case
+                        assert np.all(np.diff(got["distance"][i][:n].astype(np.float64)) >= 0), case
+                        for lab, d in zip(got["label"][i][:n].tolist(), got["distance"][i][:n].view(np.uint32).tolist()):
+                            if lab in want:
+                                hit += 1
+                                assert d == want[lab], case
+                    valid = int((exact["label"] != lib.INVALID_LABEL).sum())
+                    assert hit >= 0.99 * valid, (case, hit, valid)
+    dev.set_option("storage", 0)
