@@ -152,3 +152,34 @@ def test_deletions_and_errors(lib):
     with pytest.raises(ValueError):
         srv.submit(Q[:2], 10, 10, np.ones(10_000, bool))  # more labels than the server was sized for
     srv.close()
+
+
+def test_deletions_after_graphs_were_captured(lib):
+    """A live server with graphs: unfiltered requests before any deletion,
+    after a few (the side-list kernel, a graph of its own) and after many
+    (past 1 in 16: the filtered kernel) equal search_batch at each stage and
+    never return a deleted label."""
+    import workloads
+
+    X = workloads.mixture_128(6_000, seed=33)
+    Q = workloads.mixture_128(300, seed=1033)
+    ix = lib.Index(128, "l2")
+    ix.add(X)
+    dev = ix.device()
+    srv = lib.Server(dev, slots=2, max_batch=512, graphs=True)
+    deleted = set()
+    for stage, step in enumerate((0, 97, 7)):
+        if step:
+            for label in range(stage, 6_000, step):
+                ix.mark_deleted(label)
+                deleted.add(label)
+        print("stage", stage, "deleted", len(deleted), flush=True)
+        for replay in range(2):
+            ts = [srv.submit(Q[i * 50:(i + 1) * 50], 10, ef) for i, ef in enumerate((32, 64, 128, 200, 300, 500))]
+            for i, t in enumerate(ts):
+                rows, counts = srv.wait(t)
+                want, wc = dev.search_rows(Q[i * 50:(i + 1) * 50], 10, (32, 64, 128, 200, 300, 500)[i])
+                assert rows_equal(rows, want) and np.array_equal(counts, wc), (stage, replay, i)
+                assert not (set(rows["label"].ravel().tolist()) & deleted), (stage, replay, i)
+    assert srv.stats()["graph_launches"] > 0
+    srv.close()
