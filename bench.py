@@ -385,6 +385,8 @@ def main() -> int:
     ap.add_argument("--index", default="", help="reuse/save the built index at this path")
     ap.add_argument("--profile", action="store_true",
                     help="profiling run: no ground truth, latency or CPU baseline; one line with the timings")
+    ap.add_argument("--deleted", type=float, default=0.0, metavar="FRAC",
+                    help="profiling run only: soft-delete this fraction of the base (seeded) before the sweep")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="device-layer option for every search (e.g. hash_log2=12)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -450,6 +452,13 @@ def main() -> int:
         build_s = time.perf_counter() - t0
         log(f"built {args.n} x 128 on {threads} threads in {build_s:.1f}s (max level {ix.max_level})")
         ix.save(index_path)
+
+    if args.deleted > 0:
+        if not args.profile:
+            raise SystemExit("--deleted is for profiling runs (--profile)")
+        for lab in np.random.default_rng(77).permutation(len(ix))[: int(len(ix) * args.deleted)]:
+            ix.mark_deleted(int(lab))
+        log(f"soft-deleted {int(len(ix) * args.deleted)} of {len(ix)} labels")
 
     t0 = time.perf_counter()
     truth, truth_check, brute = (None, None, None) if args.profile else ground_truth(X, Q, k)
