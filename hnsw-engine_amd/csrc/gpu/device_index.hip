@@ -5,16 +5,19 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../../include/hnswgpu.h"
 #include "../error.hpp"
+#include "exact_scale.hpp"
 #include "index_state.hpp"
 #include "launch.hpp"
 
@@ -147,10 +150,12 @@ uint32_t resident_waves(size_t lds_per_wave, uint32_t wpb, uint32_t waves_per_cu
   return 256u * std::max<uint32_t>(wpb, std::min(by_lds, waves_per_cu));
 }
 
-/// Device-side search on device buffers (caller holds h.mu).
-void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
-                          const uint64_t* d_fw, uint64_t fsize, int by_id, hnswgpu_neighbor* d_out,
-                          uint32_t* d_counts, hipStream_t st) {
+
+/// Device-side search on device buffers with context c (caller holds h.mu
+/// shared). qscale: cosine scales computed by the host, or null.
+void search_device(hnswgpu_index& h, hnswamd::dev::CallCtx& c, const float* d_qs, const float* d_qscale,
+                   uint64_t nq, uint64_t k, uint64_t ef, const uint64_t* d_fw, uint64_t fsize, int by_id,
+                   hnswgpu_neighbor* d_out, uint32_t* d_counts, hipStream_t st) {
   const uint64_t ef_eff = hnswamd::dev::effective_ef(h, k, ef);
   if (ef_eff > 0xFFFFFFFFull || k > 0xFFFFFFFFull || nq > 0xFFFFFFFFull) {
     throw std::invalid_argument("search_batch: size out of range");
@@ -158,22 +163,24 @@ void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint
   const DevIndexView& ix = h.view;
   SearchRequest r{};
   r.queries = d_qs;
+  r.qscale = d_qscale;
   r.dim = static_cast<uint32_t>(h.dim);
   r.nq = static_cast<uint32_t>(nq);
   r.k = static_cast<uint32_t>(k);
   r.ef = static_cast<uint32_t>(ef_eff);
   r.out = reinterpret_cast<OutRow*>(d_out);
   r.counts = d_counts;
-  r.stats = h.collect_stats ? static_cast<uint32_t*>(h.main.stats.need(nq * 3 * sizeof(uint32_t))) : nullptr;
-  h.last_nq = nq;
+  r.stats = h.collect_stats ? static_cast<uint32_t*>(c.sc.stats.need(nq * 3 * sizeof(uint32_t))) : nullptr;
+  c.last_nq = nq;
+  c.stats_on = h.collect_stats != 0;
   r.force_spill = h.force_spill;
   hnswamd::dev::plan_request(h, r, r.nq);
-  SearchScratch s = hnswamd::dev::bind_scratch(h, h.main, r, r.nq, h.spill_waves, st);
+  SearchScratch s = hnswamd::dev::bind_scratch(h, c.sc, r, r.nq, h.spill_waves, st);
 
   const bool filtered = d_fw != nullptr || h.num_deleted > 0;
   r.elig = nullptr;
   if (filtered) {
-    auto* e = static_cast<uint32_t*>(h.main.elig.need(((ix.count + 31) / 32) * sizeof(uint32_t)));
+    auto* e = static_cast<uint32_t*>(c.sc.elig.need(((ix.count + 31) / 32) * sizeof(uint32_t)));
     check(hnswamd::gpu::launch_eligibility(ix, static_cast<const uint32_t*>(h.deleted.p), d_fw, fsize, by_id, e, st),
           "eligibility kernel");
     r.elig = e;
@@ -183,13 +190,13 @@ void search_device_locked(hnswgpu_index& h, const float* d_qs, uint64_t nq, uint
   check(hnswamd::gpu::launch_prepare(ix, r, s, st), "prepare kernel");
   hipEvent_t b = nullptr, e = nullptr;
   if (h.kernel_timing) {
-    if (h.ev_begin == nullptr) check(hipEventCreate(&h.ev_begin), "hipEventCreate");
-    if (h.ev_end == nullptr) check(hipEventCreate(&h.ev_end), "hipEventCreate");
-    b = h.ev_begin;
-    e = h.ev_end;
+    if (c.ev_begin == nullptr) check(hipEventCreate(&c.ev_begin), "hipEventCreate");
+    if (c.ev_end == nullptr) check(hipEventCreate(&c.ev_end), "hipEventCreate");
+    b = c.ev_begin;
+    e = c.ev_end;
   }
   check(hnswamd::gpu::launch_search(ix, r, s, st, b, e), "search kernel");
-  h.timing_pending = b != nullptr;
+  c.timing_pending = b != nullptr;
 }
 
 template <typename F>
@@ -216,6 +223,50 @@ uint32_t deleted_side_list(const hnswgpu_index& h) {
   // overflows, and hands queries to the general kernel, as they grow.
   const bool few = static_cast<uint64_t>(h.num_deleted) * 16u <= h.view.count;
   return h.deleted_path == 2 || (h.deleted_path == 1 && few) ? 1u : 0u;
+}
+
+CallCtx* acquire_ctx(hnswgpu_index& h) {
+  std::unique_lock<std::mutex> lk(h.pool_mu);
+  h.pool_cv.wait(lk, [&] { return !h.idle.empty() || h.ctxs.size() < h.max_ctxs; });
+  if (!h.idle.empty()) {
+    CallCtx* c = h.idle.back();
+    h.idle.pop_back();
+    return c;
+  }
+  auto c = std::make_unique<CallCtx>();
+  check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  check(hipEventCreateWithFlags(&c->done, hipEventDisableTiming), "hipEventCreate");
+  CallCtx* p = c.get();
+  h.ctxs.push_back(std::move(c));
+  return p;
+}
+
+void release_ctx(hnswgpu_index& h, CallCtx* c) {
+  {
+    std::lock_guard<std::mutex> lk(h.pool_mu);
+    h.idle.push_back(c);
+    h.last = c;
+  }
+  h.pool_cv.notify_one();
+}
+
+void ctx_begin(CallCtx& c, hipStream_t st) {
+  if (c.used && c.last_stream != st) check(hipStreamWaitEvent(st, c.done, 0), "hipStreamWaitEvent");
+}
+
+void ctx_end(CallCtx& c, hipStream_t st) {
+  check(hipEventRecord(c.done, st), "hipEventRecord");
+  c.last_stream = st;
+  c.used = true;
+}
+
+void host_query_scales(const float* qs, uint64_t nq, uint64_t dim, float* out) {
+  for (uint64_t i = 0; i < nq; ++i) {
+    const float* x = qs + i * dim;
+    double norm = 0.0;
+    for (uint64_t j = 0; j < dim; ++j) norm += static_cast<double>(x[j]) * x[j];
+    out[i] = norm > 0.0 ? static_cast<float>(1.0 / std::sqrt(norm)) : 1.0f;
+  }
 }
 
 gpu::DevIndexView view_for(const hnswgpu_index& h, int storage) {
@@ -247,6 +298,26 @@ void build_reduced(hnswgpu_index& h, int storage) {
   check(gpu::launch_build_storage(base, storage, low, scale, norm, h.stream), "build storage");
   check(hipStreamSynchronize(h.stream), "build storage");
   red.built = true;
+}
+
+/// Replace the index's arrays in place (after add_batch): the handle stays
+/// valid, options and tuned shapes are kept, and the traversal copy in use is
+/// rebuilt from the new rows. Refused while a server is attached (its slots
+/// are sized for the old index).
+void reupload(hnswgpu_index& h, const hnswgpu_index_view& v) {
+  std::unique_lock<std::shared_mutex> lk(h.mu);
+  if (h.servers.load() != 0) throw std::invalid_argument("upload: a server is attached to this index");
+  const int storage = h.view.storage;
+  upload_view(h, v);
+  for (Reduced& red : h.reduced) red.built = false;
+  h.view.storage = 0;
+  h.view.vlow = nullptr;
+  h.view.vscale = nullptr;
+  h.view.vnorm = nullptr;
+  if (storage != 0 && h.view.count > 0) {
+    build_reduced(h, storage);
+    h.view = view_for(h, storage);
+  }
 }
 
 uint32_t ef_class_index(uint32_t ef) {
@@ -355,6 +426,7 @@ void launch_tasks(hnswgpu_index& h, Scratch& sc, const hnswgpu_task* tasks, uint
     q.elig = t.filter >= 0 ? elig + static_cast<size_t>(t.filter) * words : del_only;
     q.k = t.k;
     q.ef = static_cast<uint32_t>(ef);
+    q.has_scale = 0;  // device queries: the cosine scale is computed on the device
     stage[i] = q;
   }
   check(hipMemcpyAsync(d_tasks, stage, n * sizeof(gpu::QueryTask), hipMemcpyHostToDevice, st), "hipMemcpy tasks");
@@ -394,14 +466,25 @@ int hnswgpu_upload(const hnswgpu_index_view* view, hnswgpu_index** out) {
   });
 }
 
+int hnswgpu_reupload(hnswgpu_index* h, const hnswgpu_index_view* view) {
+  return guarded([&] {
+    if (h == nullptr || view == nullptr) throw std::invalid_argument("upload: null pointer");
+    hnswamd::dev::reupload(*h, *view);
+  });
+}
+
 int hnswgpu_free(hnswgpu_index* h) {
-  return guarded([&] { delete h; });
+  return guarded([&] {
+    if (h != nullptr && h->servers.load() != 0) throw std::invalid_argument("free: a server is attached to this index");
+    if (h != nullptr) (void)hipDeviceSynchronize();  // device-buffer searches may still use its contexts
+    delete h;
+  });
 }
 
 int hnswgpu_set_deleted(hnswgpu_index* h, uint32_t id, uint8_t deleted) {
   return guarded([&] {
     if (h == nullptr) throw std::invalid_argument("set_deleted: null handle");
-    std::lock_guard<std::mutex> lk(h->mu);
+    std::unique_lock<std::shared_mutex> lk(h->mu);
     if (id >= h->view.count) throw std::out_of_range("set_deleted: id out of range");
     const uint8_t want = deleted != 0 ? 1 : 0;
     if (h->deleted_host[id] == want) return;
@@ -423,7 +506,7 @@ int hnswgpu_set_ef(hnswgpu_index* h, uint64_t ef) {
   return guarded([&] {
     if (h == nullptr) throw std::invalid_argument("set_ef: null handle");
     if (ef == 0) throw std::invalid_argument("set_ef: ef must be > 0");
-    std::lock_guard<std::mutex> lk(h->mu);
+    std::unique_lock<std::shared_mutex> lk(h->mu);
     h->ef_search = ef;
   });
 }
@@ -434,44 +517,72 @@ namespace hnswamd {
 
 /// Host-buffer search used by the C ABI and the C++ facade. Filter words are
 /// label-indexed (a BitsetFilter) or, when by_id, indexed by internal id
-/// (any other Filter, evaluated on the host).
+/// (any other Filter, evaluated on the host). Concurrent calls on one handle
+/// run on contexts of their own and overlap on the device.
 void device_search_host(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,
                         const uint64_t* filter_words, uint64_t filter_size, int by_id, hnswgpu_neighbor* out,
                         uint32_t* counts) {
-    if (h == nullptr) throw std::invalid_argument("search_batch: null handle");
-    if (k == 0) throw std::invalid_argument("search_batch: k must be > 0");
-    if (nq == 0) return;
-    if (qs == nullptr || out == nullptr) throw std::invalid_argument("search_batch: null pointer");
-    std::lock_guard<std::mutex> lk(h->mu);
-    if (h->view.count == 0) {
-      for (uint64_t i = 0; i < nq * k; ++i) out[i] = hnswgpu_neighbor{~0ULL, __builtin_inff(), 0u};
-      if (counts != nullptr) std::fill(counts, counts + nq, 0u);
-      return;
-    }
-    hipStream_t st = h->stream;
-    auto* d_qs = static_cast<float*>(h->host_qs.need(nq * h->dim * sizeof(float)));
-    auto* d_out = static_cast<hnswgpu_neighbor*>(h->host_out.need(nq * k * sizeof(hnswgpu_neighbor)));
-    auto* d_counts = static_cast<uint32_t*>(h->host_counts.need(nq * sizeof(uint32_t)));
-    check(hipMemcpyAsync(d_qs, qs, nq * h->dim * sizeof(float), hipMemcpyHostToDevice, st), "hipMemcpy queries");
-    const uint64_t* d_fw = nullptr;
-    if (filter_words != nullptr) {
-      const uint64_t fbits = by_id ? h->view.count : filter_size;
-      const size_t fbytes = ((fbits + 63) / 64) * sizeof(uint64_t);
-      auto* f = static_cast<uint64_t*>(h->host_filter.need(fbytes));
-      if (fbytes > 0) check(hipMemcpyAsync(f, filter_words, fbytes, hipMemcpyHostToDevice, st), "hipMemcpy filter");
-      d_fw = f;
-    }
-    search_device_locked(*h, d_qs, nq, k, ef, d_fw, filter_size, by_id, d_out, d_counts, st);
-    check(hipMemcpyAsync(out, d_out, nq * k * sizeof(hnswgpu_neighbor), hipMemcpyDeviceToHost, st),
-          "hipMemcpy results");
-    if (counts != nullptr) {
-      check(hipMemcpyAsync(counts, d_counts, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "hipMemcpy counts");
-    }
-    uint32_t ctr[8] = {};
-    check(hipMemcpyAsync(ctr, h->main.counters.p, sizeof(ctr), hipMemcpyDeviceToHost, st), "hipMemcpy counters");
-    check(hipStreamSynchronize(st), "search_batch");
-    h->last_overflow = ctr[1];
-    std::copy(ctr, ctr + 8, h->last_counters);
+  if (h == nullptr) throw std::invalid_argument("search_batch: null handle");
+  if (k == 0) throw std::invalid_argument("search_batch: k must be > 0");
+  if (nq == 0) return;
+  if (qs == nullptr || out == nullptr) throw std::invalid_argument("search_batch: null pointer");
+  std::shared_lock<std::shared_mutex> lk(h->mu);
+  if (h->view.count == 0) {
+    for (uint64_t i = 0; i < nq * k; ++i) out[i] = hnswgpu_neighbor{~0ULL, __builtin_inff(), 0u};
+    if (counts != nullptr) std::fill(counts, counts + nq, 0u);
+    return;
+  }
+  dev::CtxLease lease(*h);
+  dev::CallCtx& c = *lease.c;
+  hipStream_t st = c.stream;
+  dev::ctx_begin(c, st);
+  auto* d_qs = static_cast<float*>(c.qs.need(nq * h->dim * sizeof(float)));
+  auto* d_out = static_cast<hnswgpu_neighbor*>(c.out.need(nq * k * sizeof(hnswgpu_neighbor)));
+  auto* d_counts = static_cast<uint32_t*>(c.counts.need(nq * sizeof(uint32_t)));
+  check(hipMemcpyAsync(d_qs, qs, nq * h->dim * sizeof(float), hipMemcpyHostToDevice, st), "hipMemcpy queries");
+  const float* d_scale = nullptr;
+  std::vector<float> scales;
+  if (h->view.metric == 2) {
+    // Cosine: the query scale in double on the host, as the reference computes it.
+    scales.resize(nq);
+    dev::host_query_scales(qs, nq, h->dim, scales.data());
+    auto* f = static_cast<float*>(c.qscale.need(nq * sizeof(float)));
+    check(hipMemcpyAsync(f, scales.data(), nq * sizeof(float), hipMemcpyHostToDevice, st), "hipMemcpy scales");
+    d_scale = f;
+  }
+  const uint64_t* d_fw = nullptr;
+  if (filter_words != nullptr) {
+    const uint64_t fbits = by_id ? h->view.count : filter_size;
+    const size_t fbytes = ((fbits + 63) / 64) * sizeof(uint64_t);
+    auto* f = static_cast<uint64_t*>(c.filter.need(fbytes));
+    if (fbytes > 0) check(hipMemcpyAsync(f, filter_words, fbytes, hipMemcpyHostToDevice, st), "hipMemcpy filter");
+    d_fw = f;
+  }
+  search_device(*h, c, d_qs, d_scale, nq, k, ef, d_fw, filter_size, by_id, d_out, d_counts, st);
+  check(hipMemcpyAsync(out, d_out, nq * k * sizeof(hnswgpu_neighbor), hipMemcpyDeviceToHost, st),
+        "hipMemcpy results");
+  if (counts != nullptr) {
+    check(hipMemcpyAsync(counts, d_counts, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "hipMemcpy counts");
+  }
+  dev::ctx_end(c, st);
+  check(hipStreamSynchronize(st), "search_batch");
+}
+
+/// Device-buffer search (asynchronous on `stream`).
+void device_search_dev(hnswgpu_index* h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
+                       const uint64_t* d_words, uint64_t filter_size, int by_id, hnswgpu_neighbor* d_out,
+                       uint32_t* d_counts, void* stream) {
+  if (h == nullptr) throw std::invalid_argument("search_batch: null handle");
+  if (k == 0) throw std::invalid_argument("search_batch: k must be > 0");
+  if (nq == 0) return;
+  if (d_qs == nullptr || d_out == nullptr) throw std::invalid_argument("search_batch: null pointer");
+  std::shared_lock<std::shared_mutex> lk(h->mu);
+  if (h->view.count == 0) throw std::invalid_argument("search_batch: empty index on the device path");
+  dev::CtxLease lease(*h);
+  const auto st = static_cast<hipStream_t>(stream);
+  dev::ctx_begin(*lease.c, st);
+  search_device(*h, *lease.c, d_qs, nullptr, nq, k, ef, d_words, filter_size, by_id, d_out, d_counts, st);
+  dev::ctx_end(*lease.c, st);
 }
 
 }  // namespace hnswamd
@@ -486,18 +597,27 @@ int hnswgpu_search_batch(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_
   });
 }
 
+int hnswgpu_search_batch_ids(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,
+                             const uint64_t* id_words, hnswgpu_neighbor* out, uint32_t* counts,
+                             uint64_t /*num_threads*/) {
+  return guarded([&] {
+    hnswamd::device_search_host(h, qs, nq, k, ef, id_words, 0, 1, out, counts);
+  });
+}
+
 int hnswgpu_search_batch_device(hnswgpu_index* h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
                                 const uint64_t* d_filter_words, uint64_t filter_size,
                                 hnswgpu_neighbor* d_out, uint32_t* d_counts, void* stream) {
   return guarded([&] {
-    if (h == nullptr) throw std::invalid_argument("search_batch: null handle");
-    if (k == 0) throw std::invalid_argument("search_batch: k must be > 0");
-    if (nq == 0) return;
-    if (d_qs == nullptr || d_out == nullptr) throw std::invalid_argument("search_batch: null pointer");
-    std::lock_guard<std::mutex> lk(h->mu);
-    if (h->view.count == 0) throw std::invalid_argument("search_batch: empty index on the device path");
-    search_device_locked(*h, d_qs, nq, k, ef, d_filter_words, filter_size, 0, d_out, d_counts,
-                         static_cast<hipStream_t>(stream));
+    hnswamd::device_search_dev(h, d_qs, nq, k, ef, d_filter_words, filter_size, 0, d_out, d_counts, stream);
+  });
+}
+
+int hnswgpu_search_batch_ids_device(hnswgpu_index* h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
+                                    const uint64_t* d_id_words, hnswgpu_neighbor* d_out, uint32_t* d_counts,
+                                    void* stream) {
+  return guarded([&] {
+    hnswamd::device_search_dev(h, d_qs, nq, k, ef, d_id_words, 0, 1, d_out, d_counts, stream);
   });
 }
 
@@ -506,16 +626,27 @@ int hnswgpu_search_tasks_device(hnswgpu_index* h, const hnswgpu_task* tasks, uin
   return guarded([&] {
     if (h == nullptr) throw std::invalid_argument("search_tasks: null handle");
     if (ntasks == 0) return;
-    std::lock_guard<std::mutex> lk(h->mu);
-    // The pinned staging of the previous call may still be in its copy.
-    if (h->task_copied == nullptr) check(hipEventCreateWithFlags(&h->task_copied, hipEventDisableTiming), "hipEventCreate");
-    check(hipEventSynchronize(h->task_copied), "hipEventSynchronize");
-    auto* stage = static_cast<QueryTask*>(h->task_host.need(ntasks * sizeof(QueryTask)));
-    auto* d_tasks = static_cast<QueryTask*>(h->task_dev.need(ntasks * sizeof(QueryTask)));
+    std::shared_lock<std::shared_mutex> lk(h->mu);
+    hnswamd::dev::CtxLease lease(*h);
+    hnswamd::dev::CallCtx& c = *lease.c;
+    // The pinned task staging of this context's previous call may still be in its copy.
+    if (c.used) check(hipEventSynchronize(c.done), "hipEventSynchronize");
+    auto* stage = static_cast<QueryTask*>(c.task_host.need(ntasks * sizeof(QueryTask)));
+    auto* d_tasks = static_cast<QueryTask*>(c.task_dev.need(ntasks * sizeof(QueryTask)));
     const auto st = static_cast<hipStream_t>(stream);
-    hnswamd::dev::launch_tasks(*h, h->main, tasks, ntasks, filters, nfilters, storage < 0 ? h->view.storage : storage,
+    hnswamd::dev::ctx_begin(c, st);
+    hnswamd::dev::launch_tasks(*h, c.sc, tasks, ntasks, filters, nfilters, storage < 0 ? h->view.storage : storage,
                                stage, d_tasks, h->spill_waves, st);
-    check(hipEventRecord(h->task_copied, st), "hipEventRecord");
+    hnswamd::dev::ctx_end(c, st);
+  });
+}
+
+int hnswgpu_query_scale(const float* x, uint64_t dim, float* out) {
+  return guarded([&] {
+    if (x == nullptr || out == nullptr) throw std::invalid_argument("query_scale: null pointer");
+    if (dim > 4096) throw std::invalid_argument("query_scale: dim must be <= 4096");
+    const uint32_t bits = hnswamd::exact::inv_norm_bits(x, static_cast<uint32_t>(dim));
+    std::memcpy(out, &bits, sizeof(bits));
   });
 }
 
@@ -524,12 +655,15 @@ int hnswgpu_autotune(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k,
   return guarded([&] {
     if (h == nullptr || qs == nullptr || efs == nullptr) throw std::invalid_argument("autotune: null pointer");
     if (nq == 0 || k == 0 || nefs == 0) throw std::invalid_argument("autotune: empty sample");
-    std::lock_guard<std::mutex> lk(h->mu);
+    std::unique_lock<std::shared_mutex> lk(h->mu);
     if (h->view.count == 0) throw std::invalid_argument("autotune: empty index");
     if (nq * h->dim > (1ull << 31) || nq * k > (1ull << 31)) throw std::invalid_argument("autotune: sample too large");
-    hipStream_t st = h->stream;
-    auto* d_qs = static_cast<float*>(h->host_qs.need(nq * h->dim * sizeof(float)));
-    auto* d_out = static_cast<hnswgpu_neighbor*>(h->host_out.need(nq * k * sizeof(hnswgpu_neighbor)));
+    hnswamd::dev::CtxLease lease(*h);
+    hnswamd::dev::CallCtx& c = *lease.c;
+    hipStream_t st = c.stream;
+    hnswamd::dev::ctx_begin(c, st);
+    auto* d_qs = static_cast<float*>(c.qs.need(nq * h->dim * sizeof(float)));
+    auto* d_out = static_cast<hnswgpu_neighbor*>(c.out.need(nq * k * sizeof(hnswgpu_neighbor)));
     check(hipMemcpyAsync(d_qs, qs, nq * h->dim * sizeof(float), hipMemcpyHostToDevice, st), "hipMemcpy queries");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     check(hipEventCreate(&e0), "hipEventCreate");
@@ -555,9 +689,9 @@ int hnswgpu_autotune(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k,
               cand.generic_gather = gather;
               h->tuned[cls] = cand;
               h->use_tuned = 1;
-              search_device_locked(*h, d_qs, nq, k, ef, nullptr, 0, 0, d_out, nullptr, st);  // warm-up
+              search_device(*h, c, d_qs, nullptr, nq, k, ef, nullptr, 0, 0, d_out, nullptr, st);  // warm-up
               check(hipEventRecord(e0, st), "hipEventRecord");
-              search_device_locked(*h, d_qs, nq, k, ef, nullptr, 0, 0, d_out, nullptr, st);
+              search_device(*h, c, d_qs, nullptr, nq, k, ef, nullptr, 0, 0, d_out, nullptr, st);
               check(hipEventRecord(e1, st), "hipEventRecord");
               check(hipEventSynchronize(e1), "hipEventSynchronize");
               float ms = 0.0f;
@@ -582,6 +716,7 @@ int hnswgpu_autotune(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k,
       (void)hipEventDestroy(e1);
       throw;
     }
+    hnswamd::dev::ctx_end(c, st);
     h->use_tuned = 1;
     h->kernel_timing = saved_timing;
     (void)hipEventDestroy(e0);
@@ -597,7 +732,7 @@ int hnswgpu_autotune(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k,
 int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
   return guarded([&] {
     if (h == nullptr || name == nullptr) throw std::invalid_argument("set_option: null pointer");
-    std::lock_guard<std::mutex> lk(h->mu);
+    std::unique_lock<std::shared_mutex> lk(h->mu);
     const std::string n(name);
     if (n == "hash_log2") {
       if (value != 0 && (value < 8 || value > 15)) throw std::invalid_argument("hash_log2 must be 0 or 8..15");
@@ -616,8 +751,11 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
     } else if (n == "visit_log2") {
       if (value < 8 || value > 24) throw std::invalid_argument("visit_log2 must be 8..24");
       h->visit_log2 = static_cast<uint32_t>(value);
-      h->main.visit_tables.release();
-      h->main.visit_tables_cleared = 0;
+      std::lock_guard<std::mutex> pk(h->pool_mu);
+      for (auto& c : h->ctxs) {
+        c->sc.visit_tables.release();
+        c->sc.visit_tables_cleared = 0;
+      }
     } else if (n == "waves_per_cu") {
       if (value < 1 || value > 32) throw std::invalid_argument("waves_per_cu must be 1..32");
       h->waves_per_cu = static_cast<uint32_t>(value);
@@ -634,6 +772,10 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
     } else if (n == "spill_waves") {
       if (value < 1 || value > 4096) throw std::invalid_argument("spill_waves must be 1..4096");
       h->spill_waves = static_cast<uint32_t>(value);
+    } else if (n == "max_contexts") {
+      if (value < 1 || value > 64) throw std::invalid_argument("max_contexts must be 1..64");
+      std::lock_guard<std::mutex> pk(h->pool_mu);
+      h->max_ctxs = static_cast<uint32_t>(value);
     } else {
       throw std::invalid_argument("unknown option " + n);
     }
@@ -643,9 +785,32 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
 int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
   return guarded([&] {
     if (h == nullptr || name == nullptr || value == nullptr) throw std::invalid_argument("get_option: null pointer");
-    std::lock_guard<std::mutex> lk(h->mu);
+    std::shared_lock<std::shared_mutex> lk(h->mu);
     const std::string n(name);
-    if (n == "hash_log2") {
+    hnswamd::dev::CallCtx* last = nullptr;
+    {
+      std::lock_guard<std::mutex> pk(h->pool_mu);
+      last = h->last;
+    }
+    if (n.rfind("last_", 0) == 0 && n != "last_kernel_ns") {
+      // Counters of the most recent search on this handle (waits for it).
+      uint32_t ctr[8] = {};
+      if (last != nullptr && last->sc.counters.p != nullptr) {
+        check(hipEventSynchronize(last->done), "hipEventSynchronize");
+        check(hipMemcpy(ctr, last->sc.counters.p, sizeof(ctr), hipMemcpyDeviceToHost), "hipMemcpy counters");
+      }
+      if (n == "last_overflow") {
+        *value = ctr[1];
+      } else if (n == "last_hbm_visited") {
+        *value = ctr[3];
+      } else if (n == "last_handoff_candidates") {
+        *value = ctr[4];
+      } else if (n == "last_handoff_visited") {
+        *value = ctr[5];
+      } else {
+        throw std::invalid_argument("unknown option " + n);
+      }
+    } else if (n == "hash_log2") {
       *value = h->hash_log2;
     } else if (n == "force_spill") {
       *value = h->force_spill;
@@ -653,31 +818,6 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
       *value = h->deleted_path;
     } else if (n == "spill_waves") {
       *value = h->spill_waves;
-    } else if (n.rfind("last_", 0) == 0 && n != "last_kernel_ns" && h->main.counters.p != nullptr) {
-      // Counters of the most recent search on this handle (waits for it).
-      check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-      check(hipMemcpy(h->last_counters, h->main.counters.p, sizeof(h->last_counters), hipMemcpyDeviceToHost),
-            "hipMemcpy counters");
-      h->last_overflow = h->last_counters[1];
-      if (n == "last_overflow") {
-        *value = h->last_counters[1];
-      } else if (n == "last_hbm_visited") {
-        *value = h->last_counters[3];
-      } else if (n == "last_handoff_candidates") {
-        *value = h->last_counters[4];
-      } else if (n == "last_handoff_visited") {
-        *value = h->last_counters[5];
-      } else {
-        throw std::invalid_argument("unknown option " + n);
-      }
-    } else if (n == "last_overflow") {
-      *value = h->last_overflow;
-    } else if (n == "last_hbm_visited") {
-      *value = h->last_counters[3];
-    } else if (n == "last_handoff_candidates") {
-      *value = h->last_counters[4];
-    } else if (n == "last_handoff_visited") {
-      *value = h->last_counters[5];
     } else if (n == "visit_log2") {
       *value = h->visit_log2;
     } else if (n == "waves_per_cu") {
@@ -688,12 +828,21 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
       *value = h->collect_stats;
     } else if (n == "kernel_timing") {
       *value = h->kernel_timing;
+    } else if (n == "max_contexts") {
+      *value = h->max_ctxs;
+    } else if (n == "servers") {
+      *value = h->servers.load();
+    } else if (n == "contexts") {
+      std::lock_guard<std::mutex> pk(h->pool_mu);
+      *value = static_cast<int64_t>(h->ctxs.size());
     } else if (n == "last_kernel_ns") {
       // Fast-kernel time of the last search, from events recorded on its stream.
-      if (!h->timing_pending) throw std::invalid_argument("last_kernel_ns: enable kernel_timing and search first");
-      check(hipEventSynchronize(h->ev_end), "hipEventSynchronize");
+      if (last == nullptr || !last->timing_pending) {
+        throw std::invalid_argument("last_kernel_ns: enable kernel_timing and search first");
+      }
+      check(hipEventSynchronize(last->ev_end), "hipEventSynchronize");
       float ms = 0.0f;
-      check(hipEventElapsedTime(&ms, h->ev_begin, h->ev_end), "hipEventElapsedTime");
+      check(hipEventElapsedTime(&ms, last->ev_begin, last->ev_end), "hipEventElapsedTime");
       *value = static_cast<int64_t>(static_cast<double>(ms) * 1e6);
     } else if (n == "generic_gather") {
       *value = h->generic_gather;
@@ -735,11 +884,18 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
 extern "C" int hnswgpu_read_stats(hnswgpu_index* h, uint32_t* out, uint64_t nq) {
   return guarded([&] {
     if (h == nullptr || out == nullptr) throw std::invalid_argument("read_stats: null pointer");
-    std::lock_guard<std::mutex> lk(h->mu);
-    if (!h->collect_stats || h->main.stats.p == nullptr) throw std::invalid_argument("read_stats: collect_stats is off");
-    if (nq > h->last_nq) throw std::invalid_argument("read_stats: more queries than the last search had");
-    check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    check(hipMemcpy(out, h->main.stats.p, nq * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy stats");
+    std::shared_lock<std::shared_mutex> lk(h->mu);
+    hnswamd::dev::CallCtx* last = nullptr;
+    {
+      std::lock_guard<std::mutex> pk(h->pool_mu);
+      last = h->last;
+    }
+    if (last == nullptr || !last->stats_on || last->sc.stats.p == nullptr) {
+      throw std::invalid_argument("read_stats: collect_stats is off");
+    }
+    if (nq > last->last_nq) throw std::invalid_argument("read_stats: more queries than the last search had");
+    check(hipEventSynchronize(last->done), "hipEventSynchronize");
+    check(hipMemcpy(out, last->sc.stats.p, nq * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy stats");
   });
 }
 
@@ -853,7 +1009,7 @@ int hnswgpu_device_info(char* name, int len, int64_t* info) {
 extern "C" int hnswgpu_get_row(hnswgpu_index* h, uint32_t id, int storage, void* out, float* scale_norm) {
   return guarded([&] {
     if (h == nullptr || out == nullptr) throw std::invalid_argument("get_row: null pointer");
-    std::lock_guard<std::mutex> lk(h->mu);
+    std::shared_lock<std::shared_mutex> lk(h->mu);
     if (id >= h->view.count) throw std::out_of_range("get_row: id out of range");
     const size_t p = h->view.padded;
     if (storage == 0) {

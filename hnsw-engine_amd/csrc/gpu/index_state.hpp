@@ -19,8 +19,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <cstdint>
+#include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -100,8 +104,47 @@ struct Reduced {
 }  // namespace dev
 }  // namespace hnswamd
 
+namespace hnswamd {
+namespace dev {
+
+/// The resources of one search call in flight: its scratch, device staging for
+/// host-buffer calls, a stream of its own (host-buffer calls run there) and an
+/// event marking the end of the last work it enqueued. A call takes a context
+/// from the handle's pool and gives it back when its work is enqueued (device
+/// buffers) or finished (host buffers); the next user of the context orders
+/// itself after that event when it runs on another stream.
+struct CallCtx {
+  Scratch sc;
+  DevBuf qs, out, counts, filter, qscale, task_dev;
+  PinnedBuf task_host;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  hipStream_t last_stream = nullptr;
+  bool used = false;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;  // fast-kernel timing (option kernel_timing)
+  bool timing_pending = false;
+  uint64_t last_nq = 0;   // queries of the last search (work counts)
+  bool stats_on = false;  // the last search recorded work counts
+
+  CallCtx() = default;
+  CallCtx(const CallCtx&) = delete;
+  CallCtx& operator=(const CallCtx&) = delete;
+  ~CallCtx() {
+    if (ev_begin != nullptr) (void)hipEventDestroy(ev_begin);
+    if (ev_end != nullptr) (void)hipEventDestroy(ev_end);
+    if (done != nullptr) (void)hipEventDestroy(done);
+    if (stream != nullptr) (void)hipStreamDestroy(stream);
+  }
+};
+
+}  // namespace dev
+}  // namespace hnswamd
+
+/// A resident index. Searches hold `mu` shared for their whole call, so
+/// concurrent callers overlap on the device, each with a context of its own;
+/// uploads, deletions, options and tuning hold it exclusively.
 struct hnswgpu_index {
-  std::mutex mu;
+  std::shared_mutex mu;
   hnswamd::gpu::DevIndexView view{};  // view.storage: the copy search_batch traverses
   uint64_t dim = 0;
   uint64_t ef_search = 10;
@@ -111,10 +154,19 @@ struct hnswgpu_index {
 
   hnswamd::dev::DevBuf vecs, l0, upper, upper_off, labels, deleted;
   hnswamd::dev::Reduced reduced[3];  // [1] bf16, [2] fp8 e4m3 + scale
-  hnswamd::dev::Scratch main;        // scratch of the synchronous entry points
-  hnswamd::dev::DevBuf host_qs, host_out, host_counts, host_filter, task_dev;
-  hnswamd::dev::PinnedBuf task_host;
-  hipEvent_t task_copied = nullptr;  // the pinned task staging may be reused after this
+
+  // Call contexts: at most max_ctxs, created on demand; `last` is the one the
+  // most recent search used (read by the last_* options and read_stats).
+  std::mutex pool_mu;
+  std::condition_variable pool_cv;
+  std::vector<std::unique_ptr<hnswamd::dev::CallCtx>> ctxs;
+  std::vector<hnswamd::dev::CallCtx*> idle;
+  hnswamd::dev::CallCtx* last = nullptr;
+  uint32_t max_ctxs = 4;
+
+  // Servers attached to this index (hnswgpu_server_create); while any is,
+  // the index cannot be re-uploaded or freed.
+  std::atomic<uint32_t> servers{0};
 
   // Launch shapes chosen by the autotuner, per ef class (64, 128, 256, 512),
   // used while use_tuned is set.
@@ -139,17 +191,10 @@ struct hnswgpu_index {
   int collect_stats = 0;
   int kernel_timing = 0;
 
-  uint32_t last_counters[8] = {};
-  uint32_t last_overflow = 0;
-  uint64_t last_nq = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  bool timing_pending = false;
+  hipStream_t stream = nullptr;  // uploads, deletions and reduced-copy builds
 
   ~hnswgpu_index() {
-    if (task_copied != nullptr) (void)hipEventDestroy(task_copied);
-    if (ev_begin != nullptr) (void)hipEventDestroy(ev_begin);
-    if (ev_end != nullptr) (void)hipEventDestroy(ev_end);
+    ctxs.clear();
     if (stream != nullptr) (void)hipStreamDestroy(stream);
   }
 };
@@ -160,6 +205,9 @@ namespace dev {
 /// The index as the kernels see it, traversing `storage` (0 fp32, 1 bf16, 2
 /// fp8); the reduced copy must be built.
 gpu::DevIndexView view_for(const hnswgpu_index& h, int storage);
+/// Replace the index's arrays in place (after an add): the handle stays valid
+/// and keeps its options. Refused while a server is attached.
+void reupload(hnswgpu_index& h, const hnswgpu_index_view& v);
 /// Build the reduced copy for `storage` if it is not built yet (synchronous).
 void build_reduced(hnswgpu_index& h, int storage);
 /// Launch shape for r.ef and nq queries: hash size, wavefronts per block and a
@@ -175,6 +223,27 @@ uint64_t effective_ef(const hnswgpu_index& h, uint64_t k, uint64_t ef);
 /// 1 when searches whose only exclusions are soft deletions run the unfiltered
 /// traversal with a side list of deleted nodes (option deleted_path).
 uint32_t deleted_side_list(const hnswgpu_index& h);
+/// Take a call context from the pool (waits while max_ctxs are in use) and
+/// give it back; the one given back last is the handle's most recent.
+CallCtx* acquire_ctx(hnswgpu_index& h);
+void release_ctx(hnswgpu_index& h, CallCtx* c);
+/// Before enqueuing on st: order st after the context's earlier work when
+/// that ran on another stream. After: record the context's event on st.
+void ctx_begin(CallCtx& c, hipStream_t st);
+void ctx_end(CallCtx& c, hipStream_t st);
+/// A context held for the scope of one call.
+struct CtxLease {
+  hnswgpu_index& h;
+  CallCtx* c;
+  explicit CtxLease(hnswgpu_index& hh) : h(hh), c(acquire_ctx(hh)) {}
+  ~CtxLease() { release_ctx(h, c); }
+  CtxLease(const CtxLease&) = delete;
+  CtxLease& operator=(const CtxLease&) = delete;
+};
+/// Cosine scale of each of nq host queries (dim floats each, row-major) as the
+/// reference's prepare_vector computes it (src/space.hpp:31-40): the sum of
+/// squares in double, in order, then float(1 / sqrt), 1 for a zero vector.
+void host_query_scales(const float* qs, uint64_t nq, uint64_t dim, float* out);
 /// Mixed launch of n tasks on `st`: eligibility bitmaps for the filters the
 /// tasks use, the task array staged in pinned `stage` and copied to
 /// `d_tasks`, then prepare + fast + general kernels with scratch `sc`.

@@ -70,6 +70,8 @@ struct QueryTask {
   const uint32_t* elig;      // id bitmap (deletions and filter combined), or null = unfiltered
   uint32_t k;
   uint32_t ef;               // clamped: max(ef or ef_search, k)
+  float scale;               // cosine: the query's 1/|q|, computed by the host (has_scale = 1)
+  uint32_t has_scale;        // 0: the device computes the scale from the query
 };
 
 struct SearchRequest {
@@ -77,6 +79,7 @@ struct SearchRequest {
                              // below are unused and ef holds the largest task ef
   const uint32_t* nq_dev;    // when set: the number of queries is read here (nq is then a capacity)
   const float* queries;      // nq x dim (device)
+  const float* qscale;       // cosine: nq scales 1/|q| computed by the host, or null (computed on the device)
   uint32_t dim;
   uint32_t nq;
   uint32_t k;

@@ -31,6 +31,7 @@
 #include <math.h>
 
 #include "device_common.hpp"
+#include "exact_scale.hpp"
 #include "launch.hpp"
 #include "quad_distance.hpp"
 #include "reduced.hpp"
@@ -511,6 +512,8 @@ struct QueryArgs {
   uint32_t* stats;
   uint32_t k;
   uint32_t ef;
+  float scale;     // cosine: 1/|q| from the host (has_scale)
+  bool has_scale;
 };
 
 template <typename T>
@@ -528,6 +531,8 @@ __device__ __forceinline__ QueryArgs query_args(const SearchRequest& r, uint32_t
     a.elig = uniform_ptr(t.elig);
     a.k = uniform(t.k);
     a.ef = uniform(t.ef);
+    a.scale = __uint_as_float(uniform(__float_as_uint(t.scale)));
+    a.has_scale = uniform(t.has_scale) != 0u;
   } else {
     a.query = r.queries + static_cast<size_t>(qi) * r.dim;
     a.out = r.out + static_cast<size_t>(qi) * r.k;
@@ -535,6 +540,8 @@ __device__ __forceinline__ QueryArgs query_args(const SearchRequest& r, uint32_t
     a.elig = r.elig;
     a.k = r.k;
     a.ef = r.ef;
+    a.has_scale = r.qscale != nullptr;
+    a.scale = a.has_scale ? r.qscale[qi] : 1.0f;
   }
   a.stats = r.stats != nullptr ? r.stats + 3 * static_cast<size_t>(qi) : nullptr;
   return a;
@@ -1277,24 +1284,26 @@ __global__ void reset_counters(uint32_t* c) {
 }
 
 /// One wavefront per query: copy into a zero-padded row, normalising for
-/// cosine with the norm summed sequentially in double (src/space.hpp:31-40).
+/// cosine (src/space.hpp:31-40). The scale 1/|q| comes from the host when it
+/// has the queries (computed there in double, as the reference does); for
+/// device queries lane 0 reproduces that double arithmetic bit for bit in
+/// integers (exact_scale.hpp), so no step uses FP64.
 __global__ void __launch_bounds__(256) prepare_queries(DevIndexView ix, SearchRequest r, SearchScratch s) {
   const int lane = lane_id();
   const uint32_t qi = blockIdx.x * 4u + (threadIdx.x >> 6);
   if (qi >= launch_nq(r)) return;
-  const float* src = query_args(r, qi).query;
+  const QueryArgs a = query_args(r, qi);
+  const float* src = a.query;
   float* dst = s.qprep + static_cast<size_t>(qi) * ix.padded;
   float inv = 1.0f;
   if (ix.metric == 2) {
-    double ss = 0.0;
-    if (lane == 0) {
-      for (uint32_t i = 0; i < r.dim; ++i) {
-        const double x = static_cast<double>(src[i]);
-        ss = ss + x * x;
-      }
+    if (a.has_scale) {
+      inv = a.scale;
+    } else {
+      uint32_t bits = 0;
+      if (lane == 0) bits = exact::inv_norm_bits(src, r.dim);
+      inv = __uint_as_float(rdlane(bits, 0));
     }
-    ss = __shfl(ss, 0);
-    inv = ss > 0.0 ? static_cast<float>(1.0 / __builtin_sqrt(ss)) : 1.0f;
   }
   for (uint32_t i = static_cast<uint32_t>(lane); i < ix.padded; i += 64) {
     dst[i] = i < r.dim ? (ix.metric == 2 ? src[i] * inv : src[i]) : 0.0f;

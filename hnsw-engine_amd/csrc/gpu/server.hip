@@ -31,6 +31,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <tuple>
 #include <string>
@@ -77,7 +78,33 @@ struct Graph {
   hipGraphExec_t exec = nullptr;
   hipGraphNode_t h2d = nullptr;
   hipGraphNode_t d2h = nullptr;
+  std::vector<uint64_t> shape;  // launch parameters baked into the graph's kernel nodes
 };
+
+void destroy_graph(Graph& g) {
+  if (g.exec != nullptr) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph != nullptr) (void)hipGraphDestroy(g.graph);
+  g.exec = nullptr;
+  g.graph = nullptr;
+}
+
+/// Every value a captured launch keeps in its kernel arguments: the launch
+/// shape, the scratch pointers (which bind_scratch may reallocate when an
+/// option or the tuned shapes change) and the index view. A graph is replayed
+/// only while these are unchanged; otherwise it is captured again.
+std::vector<uint64_t> shape_words(const hnswamd::gpu::SearchRequest& r, const hnswamd::gpu::SearchScratch& s,
+                                  const hnswamd::gpu::DevIndexView& ix, const uint32_t* del_bits) {
+  auto P = [](const void* p) { return static_cast<uint64_t>(reinterpret_cast<uintptr_t>(p)); };
+  return {P(r.tasks), P(r.nq_dev), P(r.queries), P(r.qscale), r.dim, r.nq, r.k, r.ef, P(r.elig),
+          static_cast<uint64_t>(r.deleted_only), P(r.out), P(r.counts), P(r.stats), r.hash_log2, r.waves_per_block,
+          r.blocks, static_cast<uint64_t>(r.force_spill), static_cast<uint64_t>(r.generic_gather),
+          static_cast<uint64_t>(r.wide_hash),
+          P(s.counters), P(s.visit_tables), s.visit_log2, P(s.overflow), P(s.qprep), P(s.spill_words),
+          P(s.spill_heaps), s.spill_heap_stride, s.spill_waves, s.spill_bitmap_words, P(s.cand_runs),
+          P(ix.vecs), P(ix.l0), P(ix.upper), P(ix.upper_off), P(ix.labels), ix.count, ix.padded, ix.M, ix.M0,
+          ix.entry, static_cast<uint64_t>(static_cast<int64_t>(ix.max_level)), static_cast<uint64_t>(ix.metric),
+          ix.has_dups, static_cast<uint64_t>(ix.storage), P(ix.vlow), P(ix.vscale), P(ix.vnorm), P(del_bits)};
+}
 
 struct Slot {
   hipStream_t stream = nullptr;
@@ -123,17 +150,14 @@ struct hnswgpu_server {
 
   // statistics since the last reset
   std::vector<double> latencies_ms;
-  uint64_t queries_done = 0, launches = 0, graph_launches = 0, launched_queries = 0;
+  uint64_t queries_done = 0, launches = 0, graph_launches = 0, launched_queries = 0, recaptures = 0;
   bool any_submit = false;
   Clock::time_point t_first, t_last;
   double device_busy_s = 0.0;
 
   ~hnswgpu_server() {
     for (auto& s : slots) {
-      for (auto& kv : s->graphs) {
-        if (kv.second.exec != nullptr) (void)hipGraphExecDestroy(kv.second.exec);
-        if (kv.second.graph != nullptr) (void)hipGraphDestroy(kv.second.graph);
-      }
+      for (auto& kv : s->graphs) destroy_graph(kv.second);
       if (s->done != nullptr) (void)hipEventDestroy(s->done);
       if (s->stream != nullptr) (void)hipStreamDestroy(s->stream);
     }
@@ -227,7 +251,7 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
   const auto* elig = static_cast<const uint32_t*>(slot.elig.p);
   auto* dout = static_cast<uint8_t*>(slot.out.p);
 
-  std::unique_lock<std::mutex> ilk(h.mu);  // index state (deletions, views) stays put while we read it
+  std::shared_lock<std::shared_mutex> ilk(h.mu);  // index state (deletions, views) stays put while we read it
   const hnswamd::gpu::DevIndexView ix = hnswamd::dev::view_for(h, storage);
   const bool deletions = h.num_deleted > 0;
   const uint32_t side_list = hnswamd::dev::deleted_side_list(h);
@@ -266,6 +290,7 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
   for (size_t i = 0; i < order.size(); ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
   uint32_t next_task = 0;
+  std::vector<float> scales;
   for (size_t pi : order) {
     const Piece& p = slot.pieces[pi];
     const hnswgpu_request& r = p.req->r;
@@ -274,6 +299,11 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
     const uint64_t ef = hnswamd::dev::effective_ef(h, r.k, r.ef);
     const size_t qbytes = srv.dim * sizeof(float);
     std::memcpy(base + var, r.queries + p.first * srv.dim, p.n * qbytes);
+    if (ix.metric == 2) {
+      // Cosine: the query scales in double on the host, as the reference computes them.
+      scales.resize(p.n);
+      hnswamd::dev::host_query_scales(r.queries + p.first * srv.dim, p.n, srv.dim, scales.data());
+    }
     for (uint32_t i = 0; i < p.n; ++i) {
       QueryTask q{};
       q.query = reinterpret_cast<const float*>(dbase + var + i * qbytes);
@@ -282,6 +312,8 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
       q.elig = e;
       q.k = static_cast<uint32_t>(r.k);
       q.ef = static_cast<uint32_t>(ef);
+      q.has_scale = ix.metric == 2 ? 1u : 0u;
+      q.scale = ix.metric == 2 ? scales[i] : 1.0f;
       tasks[next_task++] = q;
     }
     var += p.n * qbytes;
@@ -315,7 +347,16 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
     check(hipMemcpyAsync(slot.result.p, slot.out.p, out_n, hipMemcpyDeviceToHost, slot.stream), "hipMemcpy out");
   };
   const auto key = std::make_tuple(storage, cls, r.deleted_only);
+  std::vector<uint64_t> shape = shape_words(r, s, ix, del_bits);
   auto it = slot.graphs.find(key);
+  if (it != slot.graphs.end() && it->second.shape != shape) {
+    // Captured with other launch parameters or scratch that has since been
+    // reallocated: capture again.
+    destroy_graph(it->second);
+    slot.graphs.erase(it);
+    it = slot.graphs.end();
+    ++srv.recaptures;
+  }
   if (c.use_graphs && it != slot.graphs.end()) {
     Graph& g = it->second;
     check(hipGraphExecMemcpyNodeSetParams1D(g.exec, g.h2d, slot.in.p, slot.stage.p, in_used, hipMemcpyHostToDevice),
@@ -358,7 +399,8 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
         throw hnswamd::DeviceError("server: captured graph lacks its copies");
       }
       check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-      slot.graphs.emplace(key, g);
+      g.shape = std::move(shape);
+      slot.graphs.emplace(key, std::move(g));
     }
   }
   check(hipEventRecord(slot.done, slot.stream), "hipEventRecord");
@@ -478,7 +520,7 @@ int hnswgpu_server_create(hnswgpu_index* h, const hnswgpu_server_config* config,
       throw std::invalid_argument("server_create: slots <= 16, max_batch <= 2^20, max_filters <= 256");
     }
     {
-      std::lock_guard<std::mutex> lk(h->mu);
+      std::unique_lock<std::shared_mutex> lk(h->mu);
       if (h->view.count == 0) throw std::invalid_argument("server_create: empty index");
       for (int st = 1; st <= 2; ++st) {
         if (c.storages & (1u << st)) hnswamd::dev::build_reduced(*h, st);
@@ -532,7 +574,7 @@ int hnswgpu_server_create(hnswgpu_index* h, const hnswgpu_server_config* config,
       slot->elig.need((c.max_filters + 1) * srv->words * sizeof(uint32_t));
       // Scratch for the largest launch shape, allocated and cleared now (not
       // inside a captured graph).
-      std::lock_guard<std::mutex> lk(h->mu);
+      std::shared_lock<std::shared_mutex> lk(h->mu);
       for (uint32_t cls : {64u, 128u, 256u, hnswamd::gpu::kFastMaxEf}) {
         hnswamd::gpu::SearchRequest r{};
         r.ef = cls;
@@ -544,6 +586,7 @@ int hnswgpu_server_create(hnswgpu_index* h, const hnswgpu_server_config* config,
       srv->slots.push_back(std::move(slot));
     }
     hnswgpu_server* s = srv.release();
+    h->servers.fetch_add(1);
     s->dispatcher = std::thread(dispatch_loop, s);
     s->completer = std::thread(complete_loop, s);
     *out = s;
@@ -563,7 +606,7 @@ int hnswgpu_server_submit(hnswgpu_server* s, const hnswgpu_request* req, uint64_
     auto q = std::make_shared<Request>();
     q->r = r;
     {
-      std::lock_guard<std::mutex> ilk(s->h->mu);
+      std::shared_lock<std::shared_mutex> ilk(s->h->mu);
       q->storage = r.storage < 0 ? s->h->view.storage : r.storage;
       (void)hnswamd::dev::view_for(*s->h, q->storage);  // throws if that copy is not built
       if (hnswamd::dev::effective_ef(*s->h, r.k, r.ef) > 0xFFFFFFFFull) {
@@ -631,7 +674,8 @@ int hnswgpu_server_stats(hnswgpu_server* s, double* out, uint64_t n) {
                         mean,
                         static_cast<double>(s->cfg.slots),
                         static_cast<double>(s->slot_bytes),
-                        static_cast<double>(s->graph_launches)};
+                        static_cast<double>(s->graph_launches),
+                        static_cast<double>(s->recaptures)};
     for (uint64_t i = 0; i < n && i < sizeof(v) / sizeof(v[0]); ++i) out[i] = v[i];
   });
 }
@@ -641,7 +685,7 @@ int hnswgpu_server_reset_stats(hnswgpu_server* s) {
     if (s == nullptr) throw std::invalid_argument("server_reset_stats: null server");
     std::lock_guard<std::mutex> lk(s->mu);
     s->latencies_ms.clear();
-    s->queries_done = s->launches = s->graph_launches = s->launched_queries = 0;
+    s->queries_done = s->launches = s->graph_launches = s->launched_queries = s->recaptures = 0;
     s->any_submit = false;
   });
 }
@@ -660,7 +704,9 @@ int hnswgpu_server_free(hnswgpu_server* s) {
     if (s->dispatcher.joinable()) s->dispatcher.join();
     if (s->completer.joinable()) s->completer.join();
     for (auto& slot : s->slots) (void)hipStreamSynchronize(slot->stream);
+    hnswgpu_index* h = s->h;
     delete s;
+    h->servers.fetch_sub(1);
   });
 }
 

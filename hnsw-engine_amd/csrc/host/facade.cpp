@@ -129,14 +129,7 @@ struct Index::Impl {
     params.seed = g->params.seed;
   }
 
-  /// Upload (or re-upload after add) under the shared API lock.
-  hnswgpu_index* device() const {
-    std::lock_guard<std::mutex> lk(dev_mu);
-    if (dev != nullptr && !dev_stale) return dev;
-    if (dev != nullptr) {
-      hnswgpu_free(dev);
-      dev = nullptr;
-    }
+  hnswgpu_index_view view() const {
     const hnswamd::Graph& gg = *g;
     hnswgpu_index_view v{};
     v.dim = gg.dim;
@@ -154,10 +147,29 @@ struct Index::Impl {
     v.labels = gg.labels;
     v.levels = gg.levels;
     v.deleted = gg.deleted.data();
-    const int rc = hnswgpu_upload(&v, &dev);
+    return v;
+  }
+
+  /// Upload on first use; after an add the device copy is refreshed in place,
+  /// so a handle given out by device_handle() stays valid (with its options).
+  hnswgpu_index* device() const {
+    std::lock_guard<std::mutex> lk(dev_mu);
+    if (dev != nullptr && !dev_stale) return dev;
+    const hnswgpu_index_view v = view();
+    const int rc = dev == nullptr ? hnswgpu_upload(&v, &dev) : hnswgpu_reupload(dev, &v);
     if (rc != HNSWGPU_OK) hnswamd::throw_status(rc, hnswgpu_last_error());
     dev_stale = false;
     return dev;
+  }
+
+  /// An add changes the arrays a server's slots were sized for.
+  void refuse_if_served(const char* what) const {
+    std::lock_guard<std::mutex> lk(dev_mu);
+    if (dev == nullptr) return;
+    int64_t servers = 0;
+    const int rc = hnswgpu_get_option(dev, "servers", &servers);
+    if (rc != HNSWGPU_OK) hnswamd::throw_status(rc, hnswgpu_last_error());
+    if (servers != 0) throw std::invalid_argument(std::string(what) + ": a server is attached to this index");
   }
 
   void search_rows(const float* qs, std::size_t nq, std::size_t k, std::size_t ef, Neighbor* out,
@@ -217,6 +229,7 @@ void Index::add_batch(const float* vecs, const uint64_t* labels, std::size_t n, 
   if (n == 0) return;
   if (vecs == nullptr || labels == nullptr) throw std::invalid_argument("add_batch: null input");
   std::unique_lock<std::shared_mutex> lk(impl_->api);
+  impl_->refuse_if_served("add_batch");
   impl_->g->add_batch(vecs, labels, n, num_threads);
   std::lock_guard<std::mutex> dk(impl_->dev_mu);
   impl_->dev_stale = true;
@@ -514,6 +527,18 @@ int hnsw_index_search_batch(hnsw_index* h, const float* qs, uint64_t nq, uint64_
     } else {
       h->idx.search_batch(qs, nq, k, ef, reinterpret_cast<hnsw::Neighbor*>(out), threads, nullptr);
     }
+  });
+}
+
+int hnsw_index_search_batch_pred(hnsw_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,
+                                 hnswgpu_neighbor* out, uint64_t threads, hnsw_label_predicate allowed, void* ctx) {
+  return guarded([&] {
+    if (allowed == nullptr) {
+      h->idx.search_batch(qs, nq, k, ef, reinterpret_cast<hnsw::Neighbor*>(out), threads, nullptr);
+      return;
+    }
+    const hnsw::FunctionFilter f([allowed, ctx](uint64_t label) { return allowed(label, ctx) != 0; });
+    h->idx.search_batch(qs, nq, k, ef, reinterpret_cast<hnsw::Neighbor*>(out), threads, &f);
   });
 }
 

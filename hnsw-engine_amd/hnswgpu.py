@@ -51,6 +51,11 @@ def _load():
         "hnswgpu_set_ef": ([p, u64], C.c_int),
         "hnswgpu_search_batch": ([p, p, u64, u64, u64, p, u64, p, p, u64], C.c_int),
         "hnswgpu_search_batch_device": ([p, p, u64, u64, u64, p, u64, p, p, p], C.c_int),
+        "hnswgpu_search_batch_ids": ([p, p, u64, u64, u64, p, p, p, u64], C.c_int),
+        "hnswgpu_search_batch_ids_device": ([p, p, u64, u64, u64, p, p, p, p], C.c_int),
+        "hnswgpu_reupload": ([p, p], C.c_int),
+        "hnswgpu_query_scale": ([p, u64, p], C.c_int),
+        "hnsw_index_search_batch_pred": ([p, p, u64, u64, u64, p, u64, p, p], C.c_int),
         "hnswgpu_set_option": ([p, C.c_char_p, i64], C.c_int),
         "hnswgpu_get_option": ([p, C.c_char_p, p], C.c_int),
         "hnswgpu_read_stats": ([p, p, u64], C.c_int),
@@ -131,11 +136,35 @@ def _f32(x, dim, what):
     return a, single
 
 
+LABEL_PREDICATE = C.CFUNCTYPE(C.c_int, C.c_uint64, C.c_void_p)
+
+
+def query_scale(x) -> float:
+    """The cosine query scale 1/|x| the device computes for device-buffer
+    searches (integer emulation of the reference's double arithmetic), run on
+    the host (hnswgpu_query_scale)."""
+    a = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    out = np.zeros(1, np.float32)
+    _check(_lib.hnswgpu_query_scale(_ptr(a), a.size, _ptr(out)))
+    return out[0]
+
+
+def id_words(mask) -> np.ndarray:
+    """Words of a bitmap over internal ids from a bool mask (hnswgpu_search_batch_ids)."""
+    m = np.asarray(mask, dtype=bool).reshape(-1)
+    bits = np.zeros(max(1, (m.size + 63) // 64) * 64, dtype=np.uint64)
+    bits[:m.size] = m
+    return np.ascontiguousarray(np.bitwise_or.reduce(bits.reshape(-1, 64) << np.arange(64, dtype=np.uint64), axis=1))
+
+
 def filter_words(filter, num_labels=None):
     """(words, size) of a label bitset from a bool mask over labels, an array of
-    allowed labels, or an existing (words, size) pair (None -> (None, 0))."""
+    allowed labels, or an existing (words, size) pair (None -> (None, 0)).
+    Callables are not label bitsets: Index.search takes them as predicates."""
     if filter is None:
         return None, 0
+    if callable(filter):
+        raise TypeError("filter: a callable is a predicate, not a label bitset (use Index.search / search_rows)")
     if isinstance(filter, tuple):
         w, size = filter
         return np.ascontiguousarray(w, dtype=np.uint64), int(size)
@@ -256,7 +285,7 @@ class Request(C.Structure):
 
 
 SERVER_STATS = ["requests", "queries", "launches", "mean_batch", "p50_ms", "p99_ms", "max_ms", "qps", "mean_ms",
-                "slots", "slot_bytes", "graph_launches"]
+                "slots", "slot_bytes", "graph_launches", "recaptures"]
 
 
 class Server:
@@ -337,6 +366,17 @@ class DeviceIndex:
         counts = np.empty(nq, dtype=np.uint32)
         w, size = filter_words(filter)
         _check(_lib.hnswgpu_search_batch(self._h, _ptr(q), nq, k, ef, _ptr(w), size, _ptr(out), _ptr(counts), 0))
+        return out.reshape(nq, k), counts
+
+    def search_rows_ids(self, queries, k, ef=0, id_mask=None):
+        """search_batch with an eligibility bitmap over internal ids
+        (hnswgpu_search_batch_ids): the GPU form of any Filter."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        out = np.empty(nq * k, dtype=NEIGHBOR)
+        counts = np.empty(nq, dtype=np.uint32)
+        w = None if id_mask is None else id_words(id_mask)
+        _check(_lib.hnswgpu_search_batch_ids(self._h, _ptr(q), nq, k, ef, _ptr(w), _ptr(out), _ptr(counts), 0))
         return out.reshape(nq, k), counts
 
     def get_row(self, node_id: int, storage: int, padded_dim: int):
@@ -443,11 +483,33 @@ class Index:
         _check(_lib.hnsw_index_add_batch(self._h, _ptr(v), _ptr(labels), n, num_threads))
 
     def search_rows(self, queries, k: int = 10, ef=None, filter=None):
-        """Raw Neighbor rows (nq, k) with fields label/distance."""
+        """Raw Neighbor rows (nq, k) with fields label/distance. filter: None,
+        a bool mask over labels or an array of allowed labels (a BitsetFilter),
+        or a callable(label) -> bool (a FunctionFilter, called once per node,
+        as python/bindings.cpp:101-107 of the reference wraps callables)."""
         if k <= 0:
             raise ValueError("k must be > 0")
         q, _ = _f32(queries, self.dim, "queries")
         out = np.empty(q.shape[0] * k, dtype=NEIGHBOR)
+        if callable(filter):
+            raised = []
+
+            def allowed(label, _ctx):
+                if raised:
+                    return 0
+                try:
+                    return 1 if filter(int(label)) else 0
+                except BaseException as e:  # re-raised below, as the binding propagates it
+                    raised.append(e)
+                    return 0
+
+            cb = LABEL_PREDICATE(allowed)
+            rc = _lib.hnsw_index_search_batch_pred(self._h, _ptr(q), q.shape[0], k, 0 if ef is None else ef,
+                                                   _ptr(out), 0, cb, None)
+            if raised:
+                raise raised[0]
+            _check(rc)
+            return out.reshape(q.shape[0], k)
         w, size = filter_words(filter)
         _check(_lib.hnsw_index_search_batch(self._h, _ptr(q), q.shape[0], k, 0 if ef is None else ef, _ptr(out),
                                             0, _ptr(w), size))

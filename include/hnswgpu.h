@@ -79,7 +79,14 @@ typedef struct hnswgpu_index_view {
  * which the device layer uploads once to HBM and keeps resident. */
 int hnswgpu_upload(const hnswgpu_index_view* view, hnswgpu_index** out);
 
-/* Releases the device copy. */
+/* Replaces: the device mirror's refresh after Index::add_batch
+ * (src/index.cpp:273-306 grows the arrays the mirror copied). The arrays are
+ * replaced in place: the handle, its options and its tuned launch shapes stay
+ * valid, and a reduced traversal copy in use is rebuilt. EINVAL while a server
+ * is attached to the handle (its slots are sized for the old index). */
+int hnswgpu_reupload(hnswgpu_index* h, const hnswgpu_index_view* view);
+
+/* Releases the device copy. EINVAL while a server is attached. */
 int hnswgpu_free(hnswgpu_index* h);
 
 /* Replaces: the tombstone write of Index::mark_deleted (src/index.cpp:308-318),
@@ -101,11 +108,43 @@ int hnswgpu_search_batch(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_
                          const uint64_t* filter_words, uint64_t filter_size, hnswgpu_neighbor* out,
                          uint32_t* counts, uint64_t num_threads);
 
+/* Concurrency: searches on one handle may run from several host threads at
+ * once, as the reference's const search / search_batch may (shared lock,
+ * include/hnsw/index.hpp:31-36). Each call in flight takes a context of its
+ * own (scratch, staging, stream) from the handle's pool (option
+ * "max_contexts", default 4; further callers wait for one), so their launches
+ * overlap on the device. Uploads, deletions, options and tuning are exclusive. */
+
+/* Replaces: Index::search_batch with a Filter that is not a BitsetFilter
+ * (include/hnsw/filter.hpp:12-30: FunctionFilter or any subclass), which the
+ * reference evaluates as eligible(id) = !deleted[id] && f->allowed(labels[id])
+ * (src/index_impl.hpp:259-261). The caller evaluates its predicate once per
+ * node and passes the result as id_words: ceil(count / 64) words over internal
+ * ids (bit i set = node i may be returned; NULL = no filter). Soft deletions
+ * still apply. Otherwise as hnswgpu_search_batch. */
+int hnswgpu_search_batch_ids(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,
+                             const uint64_t* id_words, hnswgpu_neighbor* out, uint32_t* counts,
+                             uint64_t num_threads);
+
 /* Same search on device buffers, asynchronous on `stream` (a hipStream_t, NULL
- * = the default stream). Calls on one handle must be ordered (same stream). */
+ * = the default stream). Buffers must stay valid until the stream reaches the
+ * end of the call's work. */
 int hnswgpu_search_batch_device(hnswgpu_index* h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
                                 const uint64_t* d_filter_words, uint64_t filter_size,
                                 hnswgpu_neighbor* d_out, uint32_t* d_counts, void* stream);
+
+/* hnswgpu_search_batch_ids on device buffers (id_words on the device too). */
+int hnswgpu_search_batch_ids_device(hnswgpu_index* h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
+                                    const uint64_t* d_id_words, hnswgpu_neighbor* d_out, uint32_t* d_counts,
+                                    void* stream);
+
+/* The cosine query scale the device computes for device-buffer searches:
+ * float(1 / sqrt(sum of double(x[i])^2)) as the reference's prepare_vector
+ * (src/space.hpp:31-40), 1 for a zero vector, reproduced with integer
+ * arithmetic only (no FP64 on the GPU). This runs the same code on the host,
+ * for checking; *out receives the float. Host-buffer searches compute the scale
+ * on the host in double. No reference counterpart. */
+int hnswgpu_query_scale(const float* x, uint64_t dim, float* out);
 
 /* A mixed batch (no reference counterpart; it generalises search_batch, whose
  * queries share k, ef and filter): every task is one query with its own k, ef
@@ -169,7 +208,10 @@ typedef struct hnswgpu_request {
 } hnswgpu_request;
 
 /* The index must stay uploaded and unchanged (no add, no reload) while a
- * server uses it; soft deletions are allowed. */
+ * server uses it: hnswgpu_reupload, hnswgpu_free and the facade's add_batch
+ * refuse while one is attached. Soft deletions, options and tuning are
+ * allowed; a slot whose captured graph no longer matches the launch shape or
+ * scratch captures it again. */
 int hnswgpu_server_create(hnswgpu_index* h, const hnswgpu_server_config* config, hnswgpu_server** out);
 /* Queue a request (never blocks on the device); *ticket identifies it. */
 int hnswgpu_server_submit(hnswgpu_server* s, const hnswgpu_request* request, uint64_t* ticket);
@@ -180,7 +222,9 @@ int hnswgpu_server_wait(hnswgpu_server* s, uint64_t ticket);
  * [1] queries completed, [2] launches, [3] mean queries per launch,
  * [4] p50 and [5] p99 request latency (ms, submit to completion), [6] max
  * latency, [7] queries/s from the first submit to the last completion,
- * [8] mean latency, [9] slots, [10] device bytes per slot, [11] graph replays. */
+ * [8] mean latency, [9] slots, [10] device bytes per slot, [11] graph replays,
+ * [12] graphs captured again because their launch shape or scratch changed
+ * (an option, the autotuner). */
 int hnswgpu_server_stats(hnswgpu_server* s, double* out, uint64_t n);
 int hnswgpu_server_reset_stats(hnswgpu_server* s);
 /* Completes every submitted request, then stops the server. */
@@ -214,9 +258,12 @@ int hnswgpu_autotune(hnswgpu_index* h, const float* queries, uint64_t nq, uint64
  *                    for deleted nodes when at most 1 node in 16 is deleted
  *                    (default), 2 = that kernel at any deletion count
  *   "spill_waves"    wavefronts of the general kernel
+ *   "max_contexts"   searches in flight on the handle at once (1..64, default 4)
  *   "collect_stats"  1 = record per-query work counts (hnswgpu_read_stats)
  *   "kernel_timing"  1 = bracket the fast kernel with events on its stream
  * and read-only values:
+ *   "servers"        servers attached to the handle
+ *   "contexts"       search contexts created so far
  *   "last_overflow"  queries the fast kernel handed to the general kernel
  *   "last_hbm_visited", "last_handoff_candidates", "last_handoff_visited"
  *   "last_kernel_ns" fast-kernel time (needs kernel_timing; waits for it)
@@ -283,6 +330,14 @@ int hnsw_index_add_batch(hnsw_index* h, const float* vecs, const uint64_t* label
 int hnsw_index_search_batch(hnsw_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,
                             hnswgpu_neighbor* out, uint64_t num_threads, const uint64_t* filter_words,
                             uint64_t filter_size);
+/* Replaces: Index::search_batch with a FunctionFilter (include/hnsw/filter.hpp:22-30):
+ * allowed(label, ctx) != 0 admits a label. It is called once per node of the
+ * index on the calling thread (the GPU takes the result as an id bitmap), so it
+ * need not be thread-safe. allowed == NULL searches unfiltered. */
+typedef int (*hnsw_label_predicate)(uint64_t label, void* ctx);
+int hnsw_index_search_batch_pred(hnsw_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,
+                                 hnswgpu_neighbor* out, uint64_t num_threads, hnsw_label_predicate allowed,
+                                 void* ctx);
 /* Replaces: Index::search (include/hnsw/index.hpp:55-56): writes up to k rows
  * and their number to *n_out. */
 int hnsw_index_search(hnsw_index* h, const float* q, uint64_t k, uint64_t ef, const uint64_t* filter_words,

@@ -479,6 +479,31 @@ int orc_search_batch(void* h, const float* qs, uint64_t nq, uint64_t k, uint64_t
   });
 }
 
+/// Distance of query i (prepared as the search prepares it) to the node of
+/// each of its k labels, in the given summation order; +inf for an invalid or
+/// unknown label. Checks that a reduced-precision traversal returns the exact
+/// fp32 distance of every id it returns.
+int orc_label_dists(void* h, const float* qs, uint64_t nq, const uint64_t* labels, uint64_t k, int order,
+                    float* out, uint64_t threads) {
+  return guard([&] {
+    const auto& ix = *static_cast<OIndex*>(h);
+    std::vector<std::pair<uint64_t, uint32_t>> by_label(ix.count);
+    for (uint64_t i = 0; i < ix.count; ++i) by_label[i] = {ix.labels[i], static_cast<uint32_t>(i)};
+    std::sort(by_label.begin(), by_label.end());
+    for_each_parallel(nq, threads, [&](std::size_t qi, std::size_t) {
+      std::vector<float> q(ix.padded, 0.0f);
+      prepare(ix.metric, qs + qi * ix.dim, q.data(), ix.dim);
+      for (uint64_t j = 0; j < k; ++j) {
+        const uint64_t lab = labels[qi * k + j];
+        auto it = std::lower_bound(by_label.begin(), by_label.end(), std::make_pair(lab, 0u));
+        out[qi * k + j] = (it != by_label.end() && it->first == lab)
+                              ? metric_dist(ix.metric, q.data(), ix.vec(it->second), ix.padded, order)
+                              : std::numeric_limits<float>::infinity();
+      }
+    });
+  });
+}
+
 /// Exact k-NN over n unpadded rows, best k by (distance, label), filter first
 /// (src/brute_force.cpp:34-69). labels == null -> 0..n-1.
 int orc_brute_force(uint64_t dim, int metric, const float* data, const uint64_t* labels, uint64_t n,

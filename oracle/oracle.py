@@ -45,6 +45,7 @@ class Oracle:
         lib.orc_mark_deleted.argtypes = [p, u64]
         lib.orc_search_batch.argtypes = [p, p, u64, u64, u64, C.c_int, p, u64, p, p, u64]
         lib.orc_brute_force.argtypes = [u64, C.c_int, p, p, u64, p, u64, u64, C.c_int, p, u64, p, u64]
+        lib.orc_label_dists.argtypes = [p, p, u64, p, u64, C.c_int, p, u64]
         self.lib = lib
 
     def _chk(self, rc):
@@ -104,6 +105,20 @@ class OracleIndex:
         return (out, cnt.reshape(nq, 3)) if counters else out
 
 
+def _label_dists(self, queries, labels, order="avx512", threads=0):
+    """Exact distance (this summation order) of query i to the node of each
+    label in labels[i]; +inf for invalid labels."""
+    q = np.ascontiguousarray(queries, np.float32)
+    lab = np.ascontiguousarray(labels, np.uint64)
+    out = np.empty(lab.shape, np.float32)
+    self.orc._chk(self.orc.lib.orc_label_dists(self.h, _ptr(q), q.shape[0], _ptr(lab), lab.shape[1], ORDER[order],
+                                               _ptr(out), threads))
+    return out
+
+
+OracleIndex.label_dists = _label_dists
+
+
 class Reference:
     """The reference library compiled from its own sources (oracle/_ref)."""
 
@@ -122,6 +137,7 @@ class Reference:
         lib.ref_save.argtypes = [p, C.c_char_p]
         lib.ref_search_batch.argtypes = [p, p, u64, u64, u64, p, u64, p, u64]
         lib.ref_brute_force.argtypes = [u64, C.c_int, p, p, u64, p, u64, u64, p, u64, p, u64]
+        lib.ref_parse_metric.argtypes = [C.c_char_p]
         for f in (lib.ref_l2_sq, lib.ref_dot):
             f.restype, f.argtypes = C.c_float, [p, p, u64]
         self.lib = lib
@@ -132,6 +148,10 @@ class Reference:
 
     def set_isa(self, name):
         self._chk(self.lib.ref_set_isa(name.encode()))
+
+    def parse_metric(self, name):
+        """The reference's parse_metric: the metric id, or -1 if it throws."""
+        return self.lib.ref_parse_metric(name.encode())
 
     def active_isa(self):
         b = C.create_string_buffer(32)

@@ -127,6 +127,13 @@ int ref_brute_force(uint64_t dim, int metric, const float* data, const uint64_t*
   });
 }
 
+/// The reference's parse_metric: the metric id, or -1 when it throws.
+int ref_parse_metric(const char* name) {
+  int id = -1;
+  if (guard([&] { id = static_cast<int>(hnsw::parse_metric(name)); }) != 0) return -1;
+  return id;
+}
+
 /// The reference's active l2_sq / dot kernels on raw rows.
 float ref_l2_sq(const float* a, const float* b, uint64_t dim) { return hnsw::l2_sq(a, b, dim); }
 float ref_dot(const float* a, const float* b, uint64_t dim) { return hnsw::dot(a, b, dim); }

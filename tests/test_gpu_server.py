@@ -183,3 +183,39 @@ def test_deletions_after_graphs_were_captured(lib):
                 assert not (set(rows["label"].ravel().tolist()) & deleted), (stage, replay, i)
     assert srv.stats()["graph_launches"] > 0
     srv.close()
+
+
+def test_graphs_follow_option_and_tuning_changes(lib, served):
+    """Options that change the launch shape or reallocate scratch (waves per CU,
+    visited-table and hash sizes, the autotuner's shapes) between server
+    launches: a graph captured before is captured again instead of replaying
+    stale scratch pointers, and every result still equals the request alone."""
+    ix, dev, X, Q, filters = served
+    rng = np.random.default_rng(13)
+    srv = lib.Server(dev, slots=2, max_batch=512, storages=(1,))
+    settings = [{}, {"waves_per_cu": 12}, {"visit_log2": 14}, {"hash_log2": 12}, {"wide_hash": 1},
+                {"hash_log2": 0, "wide_hash": 0, "waves_per_cu": 20, "visit_log2": 15}]
+    try:
+        for step, opts in enumerate(settings + ["autotune"]):
+            if opts == "autotune":
+                dev.autotune(Q[:512], 10, [64, 128, 256, 512])
+            else:
+                for name, value in opts.items():
+                    dev.set_option(name, value)
+            print("step", step, opts, flush=True)
+            reqs = make_requests(rng, Q, 24, filters)
+            got = run_server(srv, reqs, filters)
+            for i, (r, (rows, counts)) in enumerate(zip(reqs, got)):
+                want_rows, want_counts = alone(dev, r, filters)
+                assert rows_equal(rows, want_rows), (step, i)
+                assert np.array_equal(counts, want_counts), (step, i)
+        st = srv.stats()
+        assert st["recaptures"] > 0 and st["graph_launches"] > 0
+        # The index cannot be changed under a server.
+        with pytest.raises(ValueError, match="server is attached"):
+            ix.add(X[:10])
+    finally:
+        srv.close()
+        for name, value in (("use_tuned", 0), ("waves_per_cu", 16), ("hash_log2", 0), ("visit_log2", 15),
+                            ("wide_hash", 0)):
+            dev.set_option(name, value)
