@@ -52,6 +52,7 @@ def _load():
         "hnswgpu_search_batch": ([p, p, u64, u64, u64, p, u64, p, p, u64], C.c_int),
         "hnswgpu_search_batch_device": ([p, p, u64, u64, u64, p, u64, p, p, p], C.c_int),
         "hnswgpu_search_batch_ids": ([p, p, u64, u64, u64, p, p, p, u64], C.c_int),
+        "hnswgpu_search_batch_device_noalloc": ([p, p, u64, u64, u64, p, u64, p, p, p], C.c_int),
         "hnswgpu_search_batch_ids_device": ([p, p, u64, u64, u64, p, p, p, p], C.c_int),
         "hnswgpu_reupload": ([p, p], C.c_int),
         "hnswgpu_query_scale": ([p, u64, p], C.c_int),
@@ -435,6 +436,13 @@ class DeviceIndex:
                       d_filter: int = 0, filter_size: int = 0, d_counts: int = 0) -> None:
         _check(_lib.hnswgpu_search_batch_device(self._h, d_queries, nq, k, ef, d_filter or None, filter_size,
                                                 d_out, d_counts or None, stream or None))
+
+
+    def search_device_noalloc(self, d_queries: int, nq: int, k: int, ef: int, d_out: int, stream: int = 0,
+                              d_filter: int = 0, filter_size: int = 0, d_counts: int = 0) -> None:
+        """search_device with the library's static scratch (allocates nothing)."""
+        _check(_lib.hnswgpu_search_batch_device_noalloc(self._h, d_queries, nq, k, ef, d_filter or None, filter_size,
+                                                        d_out, d_counts or None, stream or None))
 
 
 class Index:

@@ -133,6 +133,26 @@ int hnswgpu_search_batch_device(hnswgpu_index* h, const float* d_qs, uint64_t nq
                                 const uint64_t* d_filter_words, uint64_t filter_size,
                                 hnswgpu_neighbor* d_out, uint32_t* d_counts, void* stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): hnswgpu_search_batch_device with the same arguments and
+ * results, whose scratch is a fixed 96 MiB device array per GPU (allocated
+ * with the library's module, held until the process ends) instead of the
+ * handle's context pool. The call allocates nothing (no hipMalloc / hipFree,
+ * no stream or event creation) and enqueues only kernels on `stream`, so it
+ * can be captured into a graph. Limits: the per-call scratch (256 B + nq x 4
+ * + nq x padded_dim x 4 + count / 8 bytes when filtered or with deletions,
+ * plus 16 x count bytes for the general kernel) must fit in the 80 MiB after
+ * the visited tables, else EINVAL with the sizes; the grid has at most 2048
+ * wavefronts; filtered queries whose candidates outgrow their registers finish
+ * in the general kernel (same rows, slower). Every call on one GPU shares the
+ * array: calls that overlap on a GPU race, and the caller must order them. The
+ * first call on each device resolves the array's address: make it outside
+ * graph capture. */
+int hnswgpu_search_batch_device_noalloc(hnswgpu_index* h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
+                                        const uint64_t* d_filter_words, uint64_t filter_size,
+                                        hnswgpu_neighbor* d_out, uint32_t* d_counts, void* stream);
+
 /* hnswgpu_search_batch_ids on device buffers (id_words on the device too). */
 int hnswgpu_search_batch_ids_device(hnswgpu_index* h, const float* d_qs, uint64_t nq, uint64_t k, uint64_t ef,
                                     const uint64_t* d_id_words, hnswgpu_neighbor* d_out, uint32_t* d_counts,
