@@ -54,8 +54,24 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
+typedef __attribute__((address_space(1))) uint32_t global_u32;
+
+/// A 16-byte LDS read that is never cached in registers (the hash words change
+/// under concurrent compare-and-swaps of other lanes). The pointer must point
+/// into LDS: a volatile access through a generic pointer would compile to a
+/// FLAT load, whose wait also retires every global load in flight.
+__device__ __forceinline__ u32x4 lds_read16(const uint32_t* p) {
+  return *reinterpret_cast<const volatile lds_u32x4*>(reinterpret_cast<uintptr_t>(p));
+}
+
+/// Bit `id` of an eligibility bitmap in global memory (a global load, not a
+/// FLAT one: the pointer arrives through a struct and its address space is
+/// otherwise unknown to the compiler).
 __device__ __forceinline__ bool elig_bit(const uint32_t* elig, uint32_t id) {
-  return (elig[id >> 5] >> (id & 31)) & 1u;
+  const auto* g = reinterpret_cast<const global_u32*>(reinterpret_cast<uintptr_t>(elig));
+  return (g[id >> 5] >> (id & 31)) & 1u;
 }
 
 struct WaveLds {
@@ -173,64 +189,71 @@ __device__ __forceinline__ bool hash_has(const uint32_t* tab, uint32_t log2h, ui
   }
 }
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
 /// Compact LDS hash: 2^L slots of 16 bits, two per LDS word (0xFFFF empty).
 /// Ids are permuted by an odd multiplier mod 2^B (a bijection on B-bit ids);
-/// the top L bits of the permuted id give the home slot and the entry stores
-/// the other B - L bits with the distance from home, so an entry identifies
-/// its id exactly (linear probing, at most `maxp` probes). Probes read the
-/// aligned group of 8 slots (16 bytes) holding the current slot at once and
-/// scan it in registers: most lookups cost one LDS read (and one
-/// compare-and-swap to insert). The table must be 16-byte aligned, L >= 3.
+/// the top L bits of the permuted id give the home slot and the other
+/// S = B - L <= 8 bits its remainder. An entry stores the remainder with the
+/// low 7 bits of the home slot, so an id's entry is the same in every slot it
+/// may occupy (linear probing, at most `maxp` <= 127 slots past home): within
+/// that window the low home bits identify the home slot, so the entry
+/// identifies the id exactly, and never equals 0xFFFF. Probes read the aligned
+/// group of 8 slots (16 bytes) holding the current slot and test all 8 against
+/// the one expected entry with whole-word bit operations: most lookups cost one
+/// LDS read and a few instructions (and one compare-and-swap to insert). The
+/// table must be 16-byte aligned, L >= 8.
 struct Hash16 {
   uint32_t L, B, maxp;
 
-  __device__ __forceinline__ void place(uint32_t id, uint32_t& home, uint32_t& rem) const {
+  __device__ __forceinline__ void place(uint32_t id, uint32_t& home, uint32_t& key) const {
     const uint32_t x = (id * 0x9E3779B1u) & ((1u << B) - 1u);
-    home = x >> (B - L);
-    rem = x & ((1u << (B - L)) - 1u);
+    const uint32_t S = B - L;
+    home = x >> S;
+    key = ((home & 0x7Fu) << S) | (x & ((1u << S) - 1u));
   }
-  /// First slot j >= o of the group whose entry is `base + j * step` (the id
-  /// at that probe distance) or empty; 8 if none. maxj bounds j (probe limit).
-  __device__ __forceinline__ uint32_t scan(const u32x4 g, uint32_t o, uint32_t base, uint32_t step,
-                                           uint32_t maxj, bool& hit) const {
+  /// Bit j: slot j of the group holds `key` or is empty (0xFFFF). key2 = key
+  /// in both halves. Bit tricks on whole words (no per-slot compares): a half
+  /// h is non-zero iff bit 15 of ((h & 0x7FFF) + 0x7FFF) | h is set.
+  static __device__ __forceinline__ uint32_t stop_mask(const u32x4 g, uint32_t key2) {
     const uint32_t wd[4] = {g.x, g.y, g.z, g.w};
-    uint32_t stop = 8u;
-    bool h = false;
+    uint32_t ns[4];
 #pragma unroll
-    for (int j = 7; j >= 0; --j) {
-      const uint32_t e = (wd[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
-      const bool match = e == base + static_cast<uint32_t>(j) * step;
-      if (static_cast<uint32_t>(j) >= o && static_cast<uint32_t>(j) <= maxj && (match || e == 0xFFFFu)) {
-        stop = static_cast<uint32_t>(j);
-        h = match;
-      }
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t m = wd[i] ^ key2;   // zero half: the entry is key
+      const uint32_t e = ~wd[i];         // zero half: the slot is empty
+      const uint32_t nzm = ((m & 0x7FFF7FFFu) + 0x7FFF7FFFu) | m;
+      const uint32_t nze = ((e & 0x7FFF7FFFu) + 0x7FFF7FFFu) | e;
+      ns[i] = nzm & nze & 0x80008000u;   // bits 15 / 31 set: not a stop
     }
-    hit = h;
-    return stop;
+    // Slot 2i's flag to bit 2i, slot 2i + 1's to bit 16 + 2i, then the odd ones to 2i + 1.
+    const uint32_t x = (ns[0] >> 15) | (ns[1] >> 13) | (ns[2] >> 11) | (ns[3] >> 9);
+    return ~(x | (x >> 15)) & 0xFFu;
+  }
+  /// Slots of the group that lie in the probe window: j >= o (the current
+  /// slot) and at most maxp slots past home (p: probe distance of slot o).
+  __device__ __forceinline__ uint32_t window(uint32_t o, uint32_t p) const {
+    uint32_t m = (0xFFu << o) & 0xFFu;
+    const uint32_t last = maxp - p + o;  // p <= maxp
+    if (last < 7u) m &= (2u << last) - 1u;
+    return m;
   }
   /// 1: inserted, 0: present, 2: no free slot within maxp probes.
   __device__ __forceinline__ int visit(uint32_t* words, uint32_t id) const {
-    uint32_t home, rem;
-    place(id, home, rem);
-    const uint32_t smask = (1u << L) - 1u, S = B - L, step = 1u << S;
+    uint32_t home, key;
+    place(id, home, key);
+    const uint32_t key2 = key | (key << 16);
+    const uint32_t smask = (1u << L) - 1u;
     uint32_t p = 0;  // probe distance of the first slot to examine
     for (;;) {
       const uint32_t slot = (home + p) & smask;
       const uint32_t o = slot & 7u, gi = slot >> 3;
-      const u32x4 g = *reinterpret_cast<const volatile u32x4*>(words + 4 * gi);
-      // Entry of slot j of this group: ((p - o + j) << S) | rem, i.e. base + j * step.
-      const uint32_t base = ((p - o) << S) + rem;
-      const uint32_t maxj = maxp - p + o;  // p - o + j <= maxp
-      bool hit;
-      const uint32_t j = scan(g, o, base, step, maxj, hit);
-      if (j < 8u) {
-        if (hit) return 0;
+      const u32x4 g = lds_read16(words + 4 * gi);
+      const uint32_t cand = stop_mask(g, key2) & window(o, p);
+      if (cand != 0u) {
+        const uint32_t j = static_cast<uint32_t>(__builtin_ctz(cand));
         const uint32_t wi = j >> 1, sh = (j & 1u) * 16u;
         const uint32_t cur = wi == 0 ? g.x : (wi == 1 ? g.y : (wi == 2 ? g.z : g.w));
-        const uint32_t want = base + j * step;
-        const uint32_t old = atomicCAS(words + 4 * gi + wi, cur, (cur & ~(0xFFFFu << sh)) | (want << sh));
+        if (((cur >> sh) & 0xFFFFu) == key) return 0;
+        const uint32_t old = atomicCAS(words + 4 * gi + wi, cur, (cur & ~(0xFFFFu << sh)) | (key << sh));
         if (old == cur) return 1;
         p = p - o + j;  // the word changed under us: look at this slot again
         continue;
@@ -240,18 +263,22 @@ struct Hash16 {
     }
   }
   __device__ __forceinline__ bool has(const uint32_t* words, uint32_t id) const {
-    uint32_t home, rem;
-    place(id, home, rem);
-    const uint32_t smask = (1u << L) - 1u, S = B - L, step = 1u << S;
+    uint32_t home, key;
+    place(id, home, key);
+    const uint32_t key2 = key | (key << 16);
+    const uint32_t smask = (1u << L) - 1u;
     uint32_t p = 0;
     for (;;) {
       const uint32_t slot = (home + p) & smask;
       const uint32_t o = slot & 7u, gi = slot >> 3;
-      const u32x4 g = *reinterpret_cast<const volatile u32x4*>(words + 4 * gi);
-      const uint32_t base = ((p - o) << S) + rem;
-      bool hit;
-      const uint32_t j = scan(g, o, base, step, maxp - p + o, hit);
-      if (j < 8u) return hit;
+      const u32x4 g = lds_read16(words + 4 * gi);
+      const uint32_t cand = stop_mask(g, key2) & window(o, p);
+      if (cand != 0u) {
+        const uint32_t j = static_cast<uint32_t>(__builtin_ctz(cand));
+        const uint32_t wi = j >> 1, sh = (j & 1u) * 16u;
+        const uint32_t cur = wi == 0 ? g.x : (wi == 1 ? g.y : (wi == 2 ? g.z : g.w));
+        return ((cur >> sh) & 0xFFFFu) == key;
+      }
       p += 8u - o;
       if (p > maxp) return false;
     }
@@ -1109,14 +1136,14 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView 
   const WaveLds w = wave_lds(base, r.hash_log2, ix.padded);
   const size_t wave_slot = static_cast<size_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   // 16-bit LDS entries when an entry can hold the id bits beyond the home
-  // slot (B - L <= 8) with 8 bits of probe distance; else 32-bit entries.
+  // slot (B - L <= 8) beside 7 bits of the home slot; else 32-bit entries.
   const uint32_t L = r.hash_log2 + 1u;
   uint32_t B = ix.count > 1u ? 32u - static_cast<uint32_t>(__builtin_clz(ix.count - 1u)) : 1u;
   B = B < L ? L : B;
   const bool compact = r.wide_hash == 0 && B <= L + 8u;
   Visited vis{w.hash, r.hash_log2,
               s.visit_tables != nullptr ? s.visit_tables + (wave_slot << s.visit_log2) : nullptr,
-              s.visit_log2, 0u, 0u, false, compact, Hash16{L, B, 254u}, false, false};
+              s.visit_log2, 0u, 0u, false, compact, Hash16{L, B, 127u}, false, false};
   uint64_t* runs = s.cand_runs != nullptr ? s.cand_runs + wave_slot * (static_cast<size_t>(kRunSlots) * 64) : nullptr;
   const uint32_t nq = launch_nq(r);
   for (;;) {
