@@ -387,6 +387,8 @@ def main() -> int:
                     help="profiling run: no ground truth, latency or CPU baseline; one line with the timings")
     ap.add_argument("--deleted", type=float, default=0.0, metavar="FRAC",
                     help="profiling run only: soft-delete this fraction of the base (seeded) before the sweep")
+    ap.add_argument("--filter", type=float, default=0.0, metavar="DENSITY",
+                    help="profiling run only: search with a seeded label filter allowing this fraction of labels")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="device-layer option for every search (e.g. hash_log2=12)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -481,9 +483,17 @@ def main() -> int:
     d_q = hnswgpu.DeviceBuffer.of(Q)
     d_out = hnswgpu.DeviceBuffer(args.nq * k * hnswgpu.NEIGHBOR.itemsize)
     host_rows = np.empty(args.nq * k, hnswgpu.NEIGHBOR)
+    d_filter, filter_size = None, 0
+    if args.filter > 0:
+        if not args.profile:
+            raise SystemExit("--filter is for profiling runs (--profile)")
+        fw, filter_size = hnswgpu.filter_words(np.random.default_rng(78).random(len(ix)) < args.filter)
+        d_filter = hnswgpu.DeviceBuffer.of(fw)
+        log(f"label filter allowing {args.filter:.0%} of {len(ix)} labels")
 
     def step(ef):
-        dev.search_device(d_q.ptr, args.nq, k, ef, d_out.ptr, stream=stream.handle)
+        dev.search_device(d_q.ptr, args.nq, k, ef, d_out.ptr, stream=stream.handle,
+                          d_filter=d_filter.ptr if d_filter is not None else 0, filter_size=filter_size)
 
     points, per_ef = [], {}
     for ef in efs:

@@ -223,6 +223,9 @@ uint64_t effective_ef(const hnswgpu_index& h, uint64_t k, uint64_t ef);
 /// 1 when searches whose only exclusions are soft deletions run the unfiltered
 /// traversal with a side list of deleted nodes (option deleted_path).
 uint32_t deleted_side_list(const hnswgpu_index& h);
+/// The same for a filter allowing the fraction `allowed` of labels (< 0:
+/// unknown): the side list when at least 15 nodes in 16 are eligible.
+uint32_t side_list_for(const hnswgpu_index& h, double allowed);
 /// Take a call context from the pool (waits while max_ctxs are in use) and
 /// give it back; the one given back last is the handle's most recent.
 CallCtx* acquire_ctx(hnswgpu_index& h);

@@ -273,10 +273,15 @@ int hnswgpu_autotune(hnswgpu_index* h, const float* queries, uint64_t nq, uint64
  *   "use_tuned"      1 = the autotuner's launch shapes (set by hnswgpu_autotune)
  *   "storage"        traversal copy, below
  *   "force_spill"    1 = every query through the general (HBM heap) kernel
- *   "deleted_path"   searches whose only exclusions are soft deletions: 0 = the
- *                    filtered kernel, 1 = the unfiltered kernel with a side list
- *                    for deleted nodes when at most 1 node in 16 is deleted
- *                    (default), 2 = that kernel at any deletion count
+ *   "deleted_path"   searches whose eligibility bitmap excludes nodes (soft
+ *                    deletions, a filter or both): 0 = the filtered kernel,
+ *                    1 = the unfiltered kernel with a side list for the
+ *                    ineligible nodes it admits when at least 15 nodes in 16
+ *                    are eligible by the host's estimate (deletions, and
+ *                    filter words on the host), else the filtered kernel
+ *                    (default), 2 = the side list always; a live candidate
+ *                    pushed out of the 64-key side list hands its query to
+ *                    the general kernel (same rows)
  *   "spill_waves"    wavefronts of the general kernel
  *   "max_contexts"   searches in flight on the handle at once (1..64, default 4)
  *   "collect_stats"  1 = record per-query work counts (hnswgpu_read_stats)

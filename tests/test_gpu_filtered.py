@@ -5,6 +5,9 @@ no handoff to the general kernel, and the rows stay equal to the oracle's
 
 from __future__ import annotations
 
+import os
+import tempfile
+
 import numpy as np
 import pytest
 
@@ -57,3 +60,31 @@ def test_overflow_and_deletions_together(lib, built):
     slow = ix.search_rows(Q, 10, 200, filter=(w, size))
     dev.set_option("force_spill", 0)
     assert rows_equal(fast, slow)
+
+
+def test_dense_label_filter_takes_the_side_list(lib, oracle, built):
+    """A label filter allowing almost every label (here 97 %, with 1 % of the
+    nodes soft-deleted on top) runs the unfiltered traversal with a side list
+    of the admitted ineligible nodes (the host estimates the eligible fraction
+    from the filter words): rows equal the oracle's, and equal the filtered
+    kernel's (option deleted_path 0)."""
+    ix, orc, X, Q = built
+    rng = np.random.default_rng(97)
+    allow = rng.random(X.shape[0]) < 0.97
+    w, size = lib.filter_words(allow)
+    # A copy of the module's index, so that its deletions stay local.
+    tmp = tempfile.mkdtemp()
+    ix.save(os.path.join(tmp, "d.idx"))
+    ix2, orc2 = lib.Index.load(os.path.join(tmp, "d.idx")), oracle.load(os.path.join(tmp, "d.idx"))
+    for lab in rng.choice(X.shape[0], X.shape[0] // 100, replace=False).tolist():
+        ix2.mark_deleted(lab)
+        orc2.mark_deleted(lab)
+    dev = ix2.device()
+    for ef, k in ((16, 10), (64, 10), (128, 100), (256, 10), (512, 10)):
+        want = orc2.search(Q, k, ef, order="avx512", filter=(w, size))
+        dev.set_option("deleted_path", 1)
+        got = ix2.search_rows(Q, k, ef, filter=(w, size))
+        assert rows_equal(got, want), (ef, k)
+        dev.set_option("deleted_path", 0)
+        assert rows_equal(ix2.search_rows(Q, k, ef, filter=(w, size)), want), (ef, k)
+    dev.set_option("deleted_path", 1)
