@@ -35,14 +35,15 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "hnsw-engine_amd"))
-# The serving section keeps 6 launches in flight on 6 streams, beside the
-# index's stream, this script's timing stream and the default stream. HIP maps
-# streams onto a pool of hardware queues (GPU_MAX_HW_QUEUES) and serialises
-# launches of streams that share one; with a pool of 4 slots would share.
+# The serving section keeps 8 launches in flight on 8 streams, beside the
+# index's stream, the search contexts' streams, this script's timing stream and
+# the default stream. HIP maps streams onto a pool of hardware queues
+# (GPU_MAX_HW_QUEUES) and serialises launches of streams that share one; with a
+# pool of 4 slots would share.
 # Set before the HIP runtime starts (the first HIP call, after this line).
-SERVING_SLOTS = 6
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 12:
-    os.environ["GPU_MAX_HW_QUEUES"] = "12"
+SERVING_SLOTS = 8
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = "QPS at recall@10≥0.95 (1M×128 L2), GPU FP32 vs BF16+rerank vs CPU; p99 ms"
 HBM_PEAK_GBS = 8000.0
@@ -111,7 +112,7 @@ def measured_traffic(ef: int, args) -> dict:
         return {"traffic": None}
 
 
-def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3) -> dict:
+def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3, max_cost: int = 0) -> dict:
     """The serving loop under the mixed load of a production index: requests
     of log-uniform batch size 1..512, k in {1, 10, 100}, ef in {32..512},
     30 % with a 10 %-density BitsetFilter, half traversing the BF16 copy, 1 %
@@ -147,7 +148,7 @@ def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3) -> dict:
                 "launches": int(st["launches"]), "mean_queries_per_launch": round(st["mean_batch"], 1),
                 "graph_launches": int(st["graph_launches"])}
 
-    srv = hnswgpu.Server(dev, slots=slots, max_batch=4096, storages=(1,))
+    srv = hnswgpu.Server(dev, slots=slots, max_batch=4096, storages=(1,), max_cost=max_cost)
     # Warm-up: every slot captures one graph per (storage, ef class, kernel)
     # on first use, the kernel being the mixed one when a launch holds a
     # filtered request and the deletions side-list one when it holds none.
@@ -171,34 +172,43 @@ def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3) -> dict:
     closed = summary(srv.stats(), time.perf_counter() - t0, reqs)
     log(f"serving closed loop {closed}")
 
-    rate = 0.5 * closed["requests"] / closed["wall_s"]
-    reqs = make(max(100, int(rate * seconds)))
-    srv.reset_stats()
-    pending: "queue.Queue[int | None]" = queue.Queue()
+    def open_loop(rate, seconds_):
+        reqs = make(max(100, int(rate * seconds_)))
+        srv.reset_stats()
+        pending: "queue.Queue[int | None]" = queue.Queue()
 
-    def waiter():
-        while (t := pending.get()) is not None:
-            srv.wait(t)
+        def waiter():
+            while (t := pending.get()) is not None:
+                srv.wait(t)
 
-    th = threading.Thread(target=waiter)
-    th.start()
-    t0 = time.perf_counter()
-    due = t0
-    for q, k, ef, f, st in reqs:
-        due += rng.exponential(1.0 / rate)
-        while (now := time.perf_counter()) < due:
-            if due - now > 2e-4:
-                time.sleep(due - now - 1e-4)
-        pending.put(srv.submit(q, k, ef, f, storage=st))
-    pending.put(None)
-    th.join()
-    opened = summary(srv.stats(), time.perf_counter() - t0, reqs)
-    opened["offered_requests_per_s"] = round(rate, 1)
-    log(f"serving open loop {opened}")
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        due = t0
+        for q, k, ef, f, st in reqs:
+            due += rng.exponential(1.0 / rate)
+            while (now := time.perf_counter()) < due:
+                if due - now > 2e-4:
+                    time.sleep(due - now - 1e-4)
+            pending.put(srv.submit(q, k, ef, f, storage=st))
+        pending.put(None)
+        th.join()
+        out = summary(srv.stats(), time.perf_counter() - t0, reqs)
+        out["offered_requests_per_s"] = round(rate, 1)
+        return out
+
+    saturated = closed["requests"] / closed["wall_s"]
+    opened = {}
+    for frac in (0.25, 0.5, 0.75):
+        opened[str(frac)] = open_loop(frac * saturated, seconds)
+        log(f"serving open loop at {frac:.2f} of the saturated request rate {opened[str(frac)]}")
     srv.close()
     return {"mix": "batch 1..512 log-uniform, k {1,10,100}, ef {32..512}, 30% filtered (10% density), "
-                   f"50% bf16 traversal, 1% deleted; {slots} slots, max 4096 queries per launch, graphs",
-            "closed_loop": closed, "open_loop_poisson": opened}
+                   f"50% bf16 traversal, 1% deleted; {slots} slots, max 4096 queries and 2^20 cost units "
+                   "(ef / filter density) per launch, graphs",
+            "closed_loop": closed,
+            "open_loop_poisson": opened["0.5"],
+            "open_loop_by_load": opened}
 
 
 def recall_at(labels: np.ndarray, truth: np.ndarray, k: int) -> float:
@@ -231,19 +241,21 @@ def cpu_baseline(index_path: str, Q: np.ndarray, truth: np.ndarray, k: int, efs,
     per_q = (time.perf_counter() - t) / probe.shape[0]
     nq = int(min(Q.shape[0], max(500, budget_s / max(per_q * len(efs) * 2, 1e-9))))
     sample = Q[:nq]
-    points = []
+    points, rows_by_ef = [], {}
     for ef in efs:
         idx.search(sample[:200], k, ef, threads=cores)  # warm
         t = time.perf_counter()
         out = idx.search(sample, k, ef, threads=cores)
         dt = time.perf_counter() - t
+        rows_by_ef[ef] = out
         rec = recall_at(out["label"].astype(np.int64), truth[:nq], k)
         points.append((ef, rec, nq / dt))
         log(f"cpu reference ef={ef} recall={rec:.4f} qps={nq / dt:,.0f} ({cores} threads)")
     return {"value": qps_at(points), "unit": "queries/s", "cores": cores, "kind": "reference",
             "isa": ref.active_isa(),
             "sample": f"first {nq} of the {Q.shape[0]} queries, ef sweep {list(efs)}, search_batch on {cores} threads",
-            "points": [{"ef": e, "recall": round(r, 5), "qps": round(q, 1)} for e, r, q in points]}
+            "points": [{"ef": e, "recall": round(r, 5), "qps": round(q, 1)} for e, r, q in points],
+            "_rows": rows_by_ef}
 
 
 def stored_distance_error(dev, X: np.ndarray, Q: np.ndarray, storage: int, metric: str, rows: int = 2000,
@@ -285,17 +297,17 @@ def stored_distance_error(dev, X: np.ndarray, Q: np.ndarray, storage: int, metri
             "p99_rel": float(f"{np.quantile(rel, 0.99):.3e}"), "max_rel": float(f"{rel.max():.3e}")}
 
 
-def precision_workload(args) -> int:
-    """The reduced-precision configurations at their own sizes: 1M x 100 cosine with bf16 rows
-    (k 10 and 100) and 200k x 960 L2 with fp8 e4m3 rows + per-vector scale (k 10). Each sweeps ef
-    for the fp32 traversal and the reduced one (exact fp32 re-rank of the final candidates) and
-    reports QPS at recall 0.95 for both, the stored copy's distance error against FP64 and the
-    recall against GPU brute-force ground truth (FP64-checked on a sample)."""
+def reduced_precision_run(args, which: str) -> dict:
+    """A reduced-precision configuration at its own size: 1M x 100 cosine with bf16 rows (k 10 and
+    100; `which` = cosine100) or 200k x 960 L2 with fp8 e4m3 rows + per-vector scale (k 10;
+    sparse960). It sweeps ef for the fp32 traversal and the reduced one (exact fp32 re-rank of the
+    final candidates) and reports QPS at recall 0.95 for both and their ratio, the recall at every
+    ef, the stored copy's distance error against FP64, and the ground truth's agreement with FP64
+    (GPU brute force, checked in FP64 on a sample)."""
     import hnswgpu
     import workloads
 
-    log(f"device {hnswgpu.device_info()}")
-    if args.workload == "cosine100":
+    if which == "cosine100":
         n, nq, metric, storage, ks, efs = 1_000_000, 10_000, "cosine", 1, (10, 100), (32, 64, 128, 256, 512)
         X, Q = workloads.cosine_100(n, seed=3), workloads.cosine_100(nq, seed=1003)
         name = "1M x 100 cosine (1,000-centre mixture, log-normal norms), bf16 rows + fp32 re-rank"
@@ -353,7 +365,7 @@ def precision_workload(args) -> int:
                 rows = d_out.download(host_rows).reshape(nq, k)
                 rec = recall_at(rows["label"].astype(np.int64), truth, k)
                 pts.append((ef, rec, nq / (ms / 1e3)))
-                log(f"{args.workload} k={k} storage={st} ef={ef} recall={rec:.4f} qps={nq / (ms / 1e3):,.0f} step={ms:.3f}ms")
+                log(f"{which} k={k} storage={st} ef={ef} recall={rec:.4f} qps={nq / (ms / 1e3):,.0f} step={ms:.3f}ms")
             v = qps_at(pts)
             per["fp32" if st == 0 else "reduced"] = {
                 "qps_at_recall_0.95": round(v, 1) if math.isfinite(v) else None,
@@ -365,7 +377,15 @@ def precision_workload(args) -> int:
     dev.set_option("storage", storage)
     result["stored_distance_error_vs_fp64"] = stored_distance_error(dev, X, Q, storage, metric)
     dev.set_option("storage", 0)
-    print(json.dumps(result), flush=True)
+    return result
+
+
+def precision_workload(args) -> int:
+    """One JSON line for one reduced-precision configuration (--workload)."""
+    import hnswgpu
+
+    log(f"device {hnswgpu.device_info()}")
+    print(json.dumps(reduced_precision_run(args, args.workload)), flush=True)
     return 0
 
 
@@ -394,8 +414,13 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (counter calibration) and exit")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop section")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 1M x 100 cosine bf16 and 200k x 960 fp8 configurations")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
-    ap.add_argument("--serving-seconds", type=float, default=5.0)
+    ap.add_argument("--serving-seconds", type=float, default=3.0)
+    ap.add_argument("--serving-slots", type=int, default=SERVING_SLOTS)
+    ap.add_argument("--serving-max-cost", type=int, default=0, help="cost units per launch (0: the server's default)")
+    ap.add_argument("--serving-only", action="store_true", help="only the serving section, one line")
     ap.add_argument("--profile-brute", action="store_true",
                     help="profiling run of the brute force only (n x 128 data, nq queries, k), one line")
     ap.add_argument("--workload", default="mixture1m", choices=["mixture1m", "cosine100", "sparse960"],
@@ -455,6 +480,11 @@ def main() -> int:
         log(f"built {args.n} x 128 on {threads} threads in {build_s:.1f}s (max level {ix.max_level})")
         ix.save(index_path)
 
+    if args.serving_only:
+        print(json.dumps({"serving": serving_bench(ix, Q, args.serving_seconds, args.serving_slots,
+                                                   args.serving_max_cost)}), flush=True)
+        return 0
+
     if args.deleted > 0:
         if not args.profile:
             raise SystemExit("--deleted is for profiling runs (--profile)")
@@ -495,7 +525,7 @@ def main() -> int:
         dev.search_device(d_q.ptr, args.nq, k, ef, d_out.ptr, stream=stream.handle,
                           d_filter=d_filter.ptr if d_filter is not None else 0, filter_size=filter_size)
 
-    points, per_ef = [], {}
+    points, per_ef, gpu_rows = [], {}, {}
     for ef in efs:
         for _ in range(args.warmup):
             step(ef)
@@ -508,6 +538,7 @@ def main() -> int:
         stream.synchronize()
         ms = start.elapsed_ms(end) / args.steps
         rows = d_out.download(host_rows).reshape(args.nq, k)
+        gpu_rows[ef] = rows.copy()
         rec = float("nan") if truth is None else recall_at(rows["label"].astype(np.int64), truth, k)
         qps = args.nq / (ms / 1e3)
         # Fast-kernel time per launch (events on the same stream) and work counts.
@@ -584,12 +615,32 @@ def main() -> int:
     lat = np.array(lat[20:])
     del q1
 
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(index_path, Q, truth, k, efs, args.cpu_budget_s)
+    if cpu is not None:
+        # Row parity with the reference's own search_batch on the same file and queries: a row
+        # mismatches if any label or distance bit differs (the GPU computes in the AVX-512 order).
+        parity = {"against": f"reference search_batch ({cpu['isa']} kernels)", "mismatching_rows": {}}
+        for ef, ref_rows in cpu.pop("_rows").items():
+            n = ref_rows.shape[0]
+            g = gpu_rows[ef][:n]
+            bad = ~((g["label"] == ref_rows["label"]).all(1) &
+                    (g["distance"].view(np.uint32) == ref_rows["distance"].view(np.uint32)).all(1))
+            parity["mismatching_rows"][str(ef)] = int(bad.sum())
+            parity["rows_compared"] = int(n)
+        log(f"parity with the reference: {parity}")
 
     # Last: it soft-deletes 1 % of the base.
-    serving = None if args.no_serving else serving_bench(ix, Q, args.serving_seconds, SERVING_SLOTS)
+    serving = None if args.no_serving else serving_bench(ix, Q, args.serving_seconds, args.serving_slots,
+                                                         args.serving_max_cost)
+    del ix, dev
+    precision = {}
+    if not args.no_precision:
+        for which, key in (("cosine100", "cosine100_bf16"), ("sparse960", "sparse960_fp8")):
+            t0 = time.perf_counter()
+            precision[key] = reduced_precision_run(args, which)
+            log(f"{which} in {time.perf_counter() - t0:.1f}s")
 
     line = {
         "metric": METRIC,
@@ -613,10 +664,12 @@ def main() -> int:
                      "kernel": "fast_search", "at_ef": op["ef"], "kernel_ms": op["kernel_ms"],
                      "bytes_per_query": op["bytes_per_query"]},
         "cpu_baseline": cpu,
+        "parity": parity,
         "brute_force": brute,
         "autotune": tuned,
         "serving": serving,
         **reduced,
+        **precision,
         "latency_ms": {"single_query_p50": round(float(np.percentile(lat, 50)), 4),
                        "single_query_p99": round(float(np.percentile(lat, 99)), 4),
                        "at_ef": op["ef"]},

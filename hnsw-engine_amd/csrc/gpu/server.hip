@@ -56,6 +56,8 @@ struct Request {
   uint64_t ticket = 0;
   hnswgpu_request r{};
   int storage = 0;
+  double cost = 1.0;    // estimated cost of one of its queries (max(ef, k) / filter density)
+  bool heavy = false;   // cost >= heavy_cost: launched only with other heavy requests
   uint64_t placed = 0;  // queries placed in launches
   uint64_t done = 0;    // queries completed
   int status = HNSWGPU_OK;
@@ -114,6 +116,9 @@ struct Slot {
   hnswamd::dev::DevBuf in, out, elig;
   std::map<std::tuple<int, uint32_t, int>, Graph> graphs;  // (storage, ef class, deletions-only kernel)
   std::vector<Piece> pieces;
+  bool heavy = false;     // the launch holds heavy requests
+  bool launched = false;  // a launch is in flight (its completion thread owns the slot)
+  std::thread completer;
   uint32_t nq = 0;
   uint64_t rows = 0;
   std::string error;  // launch failure, reported to the launch's requests
@@ -142,11 +147,12 @@ struct hnswgpu_server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done, cv_req;
   std::deque<std::shared_ptr<Request>> pending;
-  std::deque<Slot*> free_slots, inflight;
+  std::deque<Slot*> free_slots;
+  uint32_t heavy_inflight = 0;  // launches of heavy requests in flight
   std::unordered_map<uint64_t, std::shared_ptr<Request>> requests;
   uint64_t next_ticket = 1;
   bool stopping = false;
-  std::thread dispatcher, completer;
+  std::thread dispatcher;
 
   // statistics since the last reset
   std::vector<double> latencies_ms;
@@ -186,11 +192,25 @@ std::vector<std::pair<const uint64_t*, uint64_t>> form_launch(hnswgpu_server& sr
   slot.rows = 0;
   std::vector<std::pair<const uint64_t*, uint64_t>> filters;
   size_t var_used = 0;
-  storage = srv.pending.front()->storage;
+  // The oldest request that may start a launch now: heavy ones only while a
+  // slot stays free for the others.
+  const bool heavy_ok = srv.heavy_inflight + 1 < srv.cfg.slots || srv.cfg.slots == 1;
+  const Request* first = nullptr;
+  for (const auto& p : srv.pending) {
+    if (!p->heavy || heavy_ok) {
+      first = p.get();
+      break;
+    }
+  }
+  if (first == nullptr) return {};
+  storage = first->storage;
+  slot.heavy = first->heavy;
   max_ef = 1;
+  double cost = 0.0;  // estimated cost of the launch so far
+  const double max_cost = c.max_cost != 0 ? static_cast<double>(c.max_cost) : 1e300;
   for (auto it = srv.pending.begin(); it != srv.pending.end() && slot.nq < c.max_batch;) {
     Request& q = **it;
-    if (q.storage != storage) {
+    if (q.storage != storage || q.heavy != slot.heavy) {
       ++it;
       continue;
     }
@@ -198,6 +218,11 @@ std::vector<std::pair<const uint64_t*, uint64_t>> form_launch(hnswgpu_server& sr
     const size_t qbytes = srv.dim * sizeof(float);
     uint64_t can = std::min<uint64_t>(q.r.nq - q.placed, c.max_batch - slot.nq);
     can = std::min<uint64_t>(can, (c.max_rows - slot.rows) / k);
+    // Launches are bounded by cost, so that expensive queries (large ef,
+    // sparse filters) do not make every query of the launch wait for them.
+    const double room = max_cost - cost;
+    const uint64_t by_cost = room > 0.0 ? static_cast<uint64_t>(room / q.cost) : 0;
+    can = std::min<uint64_t>(can, slot.nq == 0 ? std::max<uint64_t>(by_cost, 1) : by_cost);
     int fidx = -1;
     size_t fbytes = 0;  // words of a filter this launch does not hold yet
     if (q.r.filter_words != nullptr) {
@@ -224,6 +249,7 @@ std::vector<std::pair<const uint64_t*, uint64_t>> form_launch(hnswgpu_server& sr
     }
     slot.pieces.push_back(Piece{*it, q.placed, static_cast<uint32_t>(can), slot.nq, slot.rows});
     (void)fidx;
+    cost += static_cast<double>(can) * q.cost;
     var_used += can * qbytes;
     slot.nq += static_cast<uint32_t>(can);
     slot.rows += can * k;
@@ -265,14 +291,6 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
     var += nw * sizeof(uint64_t);
   }
   descs[c.max_filters] = FilterDesc{nullptr, 0, deletions ? 1u : 0u, 0u};
-  // Fraction of labels each filter allows (its cost grows about as 1 / density).
-  std::vector<double> density(filters.size(), 1.0);
-  for (size_t f = 0; f < filters.size(); ++f) {
-    uint64_t ones = 0;
-    const size_t nw = (filters[f].second + 63) / 64;
-    for (size_t i = 0; i < nw; ++i) ones += static_cast<uint64_t>(__builtin_popcountll(filters[f].first[i]));
-    density[f] = filters[f].second > 0 ? static_cast<double>(ones) / static_cast<double>(filters[f].second) : 1.0;
-  }
   std::vector<int> fidx(slot.pieces.size(), -1);
   std::vector<double> cost(slot.pieces.size());
   for (size_t pi = 0; pi < slot.pieces.size(); ++pi) {
@@ -280,8 +298,7 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
     for (size_t f = 0; f < filters.size(); ++f) {
       if (filters[f].first == r.filter_words && filters[f].second == r.filter_size) fidx[pi] = static_cast<int>(f);
     }
-    cost[pi] = static_cast<double>(hnswamd::dev::effective_ef(h, r.k, r.ef)) /
-               (fidx[pi] >= 0 ? std::max(density[fidx[pi]], 0.01) : 1.0);
+    cost[pi] = slot.pieces[pi].req->cost;
   }
   // The work queue runs the costliest queries first, so that no long query
   // starts at the end of the launch (longest processing time first). Each
@@ -409,7 +426,17 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
 void dispatch_loop(hnswgpu_server* srv) {
   for (;;) {
     std::unique_lock<std::mutex> lk(srv->mu);
-    srv->cv_work.wait(lk, [&] { return srv->stopping || (!srv->pending.empty() && !srv->free_slots.empty()); });
+    // A launch can start when a slot is free and some pending request may
+    // start it (a heavy one only while another slot stays free).
+    auto can_start = [&] {
+      if (srv->pending.empty() || srv->free_slots.empty()) return false;
+      if (srv->heavy_inflight + 1 < srv->cfg.slots || srv->cfg.slots == 1) return true;
+      for (const auto& p : srv->pending) {
+        if (!p->heavy) return true;
+      }
+      return false;
+    };
+    srv->cv_work.wait(lk, [&] { return srv->stopping || can_start(); });
     if (srv->stopping) return;
     Slot* slot = srv->free_slots.front();
     srv->free_slots.pop_front();
@@ -429,6 +456,7 @@ void dispatch_loop(hnswgpu_server* srv) {
     }
     ++srv->launches;
     srv->launched_queries += slot->nq;
+    if (slot->heavy) ++srv->heavy_inflight;
     lk.unlock();
     slot->error.clear();
     try {
@@ -439,17 +467,19 @@ void dispatch_loop(hnswgpu_server* srv) {
       (void)hipStreamSynchronize(slot->stream);  // whatever was enqueued before the failure
     }
     lk.lock();
-    srv->inflight.push_back(slot);
-    srv->cv_done.notify_one();
+    slot->launched = true;
+    srv->cv_done.notify_all();
   }
 }
 
-void complete_loop(hnswgpu_server* srv) {
+/// Completion of one slot's launches (each slot has its own thread, so a
+/// short launch completes as soon as it is done, whatever else is in flight):
+/// copy each request's rows to its buffers, record latencies, free the slot.
+void complete_loop(hnswgpu_server* srv, Slot* slot) {
   for (;;) {
     std::unique_lock<std::mutex> lk(srv->mu);
-    srv->cv_done.wait(lk, [&] { return !srv->inflight.empty() || (srv->stopping && srv->inflight.empty()); });
-    if (srv->inflight.empty()) return;
-    Slot* slot = srv->inflight.front();
+    srv->cv_done.wait(lk, [&] { return slot->launched || srv->stopping; });
+    if (!slot->launched) return;
     lk.unlock();
     std::string err = slot->error;
     if (err.empty()) {
@@ -483,7 +513,8 @@ void complete_loop(hnswgpu_server* srv) {
       }
     }
     slot->pieces.clear();
-    srv->inflight.pop_front();
+    if (slot->heavy) --srv->heavy_inflight;
+    slot->launched = false;
     srv->free_slots.push_back(slot);
     srv->cv_work.notify_one();
     srv->cv_req.notify_all();
@@ -516,6 +547,7 @@ int hnswgpu_server_create(hnswgpu_index* h, const hnswgpu_server_config* config,
     if (c.max_rows == 0) c.max_rows = 64ull * c.max_batch;
     if (c.max_filters == 0) c.max_filters = 8;
     if (c.spill_waves == 0) c.spill_waves = 16;
+    if (c.heavy_cost == 0) c.heavy_cost = 1024;
     if (c.slots > 16 || c.max_batch > (1u << 20) || c.max_filters > 256) {
       throw std::invalid_argument("server_create: slots <= 16, max_batch <= 2^20, max_filters <= 256");
     }
@@ -588,7 +620,7 @@ int hnswgpu_server_create(hnswgpu_index* h, const hnswgpu_server_config* config,
     hnswgpu_server* s = srv.release();
     h->servers.fetch_add(1);
     s->dispatcher = std::thread(dispatch_loop, s);
-    s->completer = std::thread(complete_loop, s);
+    for (auto& slot : s->slots) slot->completer = std::thread(complete_loop, s, slot.get());
     *out = s;
   });
 }
@@ -605,6 +637,19 @@ int hnswgpu_server_submit(hnswgpu_server* s, const hnswgpu_request* req, uint64_
     }
     auto q = std::make_shared<Request>();
     q->r = r;
+    // Cost of one query: its ef over the fraction of labels its filter allows
+    // (a filtered search expands about that many more nodes).
+    double density = 1.0;
+    if (r.filter_words != nullptr) {
+      uint64_t ones = 0;
+      const uint64_t nw = (r.filter_size + 63) / 64;
+      for (uint64_t i = 0; i < nw; ++i) {
+        uint64_t w = r.filter_words[i];
+        if (i + 1 == nw && r.filter_size % 64 != 0) w &= (1ULL << (r.filter_size % 64)) - 1;
+        ones += static_cast<uint64_t>(__builtin_popcountll(w));
+      }
+      density = r.filter_size > 0 ? static_cast<double>(ones) / static_cast<double>(r.filter_size) : 0.0;
+    }
     {
       std::shared_lock<std::shared_mutex> ilk(s->h->mu);
       q->storage = r.storage < 0 ? s->h->view.storage : r.storage;
@@ -612,6 +657,8 @@ int hnswgpu_server_submit(hnswgpu_server* s, const hnswgpu_request* req, uint64_
       if (hnswamd::dev::effective_ef(*s->h, r.k, r.ef) > 0xFFFFFFFFull) {
         throw std::invalid_argument("server_submit: ef out of range");
       }
+      q->cost = static_cast<double>(hnswamd::dev::effective_ef(*s->h, r.k, r.ef)) / std::max(density, 0.02);
+      q->heavy = q->cost >= static_cast<double>(s->cfg.heavy_cost);
     }
     q->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
@@ -702,7 +749,9 @@ int hnswgpu_server_free(hnswgpu_server* s) {
     s->cv_work.notify_all();
     s->cv_done.notify_all();
     if (s->dispatcher.joinable()) s->dispatcher.join();
-    if (s->completer.joinable()) s->completer.join();
+    for (auto& slot : s->slots) {
+      if (slot->completer.joinable()) slot->completer.join();
+    }
     for (auto& slot : s->slots) (void)hipStreamSynchronize(slot->stream);
     hnswgpu_index* h = s->h;
     delete s;

@@ -21,10 +21,10 @@ import numpy as np
 # maps streams onto a pool of hardware queues (4 per process by default) and
 # launches of streams that share a queue run one after another, so with the
 # index's own stream and the default stream beside three slots, two slots can
-# end up serialised. Unless the caller chose a value, ask for 12 queues (room
-# for 8 slots); the HIP runtime reads this once, when it starts (the first HIP
-# call).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "12")
+# end up serialised. Unless the caller chose a value, ask for 16 queues (room
+# for 8 slots beside the search contexts' streams); the HIP runtime reads this
+# once, when it starts (the first HIP call).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 _HERE = Path(__file__).resolve().parent
 LIB_PATH = _HERE / "libhnswgpu.so"
@@ -276,7 +276,8 @@ class FilterRef(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("max_batch", C.c_uint32), ("max_rows", C.c_uint64),
                 ("max_filters", C.c_uint32), ("use_graphs", C.c_uint32), ("max_filter_labels", C.c_uint64),
-                ("hbm_budget", C.c_uint64), ("spill_waves", C.c_uint32), ("storages", C.c_uint32)]
+                ("hbm_budget", C.c_uint64), ("spill_waves", C.c_uint32), ("storages", C.c_uint32),
+                ("max_cost", C.c_uint64), ("heavy_cost", C.c_uint64)]
 
 
 class Request(C.Structure):
@@ -296,9 +297,9 @@ class Server:
 
     def __init__(self, device: "DeviceIndex", slots: int = 2, max_batch: int = 4096, max_rows: int = 0,
                  max_filters: int = 8, graphs: bool = True, max_filter_labels: int = 0, hbm_budget: int = 0,
-                 spill_waves: int = 0, storages=()):
+                 spill_waves: int = 0, storages=(), max_cost: int = 0, heavy_cost: int = 0):
         cfg = ServerConfig(slots, max_batch, max_rows, max_filters, 1 if graphs else 0, max_filter_labels,
-                           hbm_budget, spill_waves, sum(1 << int(st) for st in storages))
+                           hbm_budget, spill_waves, sum(1 << int(st) for st in storages), max_cost, heavy_cost)
         h = C.c_void_p()
         _check(_lib.hnswgpu_server_create(device._h, C.byref(cfg), C.byref(h)))
         self._h, self._device, self._live = h, device, {}

@@ -209,6 +209,14 @@ typedef struct hnswgpu_server_config {
                                   the slot count is reduced to fit */
   uint32_t spill_waves;        /* general-kernel wavefronts per slot (0 = 16) */
   uint32_t storages;           /* bit s set: build the reduced copy s (1 bf16, 2 fp8) now */
+  uint64_t max_cost;           /* estimated cost per launch, in units of one unfiltered query at
+                                  ef 1: a query costs max(ef, k) / (fraction of labels its filter
+                                  allows, at least 1/50); 0 = no bound; a launch always takes at
+                                  least one query */
+  uint64_t heavy_cost;         /* requests whose queries cost at least this much (0 = 1024, e.g. a
+                                  10 %-density filter at ef 128) form launches of their own, which
+                                  hold at most all but one slot, so that cheap requests never wait
+                                  for them */
 } hnswgpu_server_config;
 
 /* One request: search_batch's arguments on host buffers, which must stay valid
