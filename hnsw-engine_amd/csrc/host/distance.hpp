@@ -21,8 +21,10 @@ namespace hnswamd {
 
 enum Metric : int { kL2 = 0, kInnerProduct = 1, kCosine = 2 };
 
+// Always inlined, so that a caller compiled for a wider ISA (the builder's
+// AVX-512 clone) runs the loops in its own registers.
 template <bool kSquaredDiff>
-inline float ordered_sum(const float* __restrict a, const float* __restrict b, std::size_t n) noexcept {
+[[gnu::always_inline]] inline float ordered_sum(const float* __restrict a, const float* __restrict b, std::size_t n) noexcept {
   float acc[4][16] = {};
   std::size_t i = 0;
   for (; i + 64 <= n; i += 64) {

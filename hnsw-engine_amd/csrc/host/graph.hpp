@@ -8,12 +8,11 @@
 // The state either owns its arrays (built here, or loaded by reading the file)
 // or points into a read-only file mapping (mmap load).
 
-#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <cstdlib>
+#include <functional>
 #include <memory>
-#include <mutex>
 #include <new>
 #include <string>
 #include <unordered_map>
@@ -25,7 +24,6 @@ inline constexpr std::size_t kRowAlign = 64;      // bytes; file sections and ro
 inline constexpr std::size_t kPadFloats = 16;     // rows padded to 16 floats
 inline constexpr uint32_t kNoNode = 0xFFFFFFFFu;  // empty entry point
 inline constexpr int kMaxLevel = 64;
-inline constexpr std::size_t kNodeLocks = 4096;   // striped per-node locks (power of two)
 
 inline std::size_t round_up(std::size_t v, std::size_t m) { return (v + m - 1) / m * m; }
 
@@ -74,6 +72,29 @@ struct BuildParams {
   uint64_t seed = 100;
 };
 
+struct Graph;
+class Team;
+
+/// A node at a distance; ordered by (distance, id) everywhere.
+struct Key {
+  float d;
+  uint32_t id;
+};
+
+/// The graph as it stood before an insertion wave: its entry point and top level.
+struct WaveFrame {
+  uint32_t entry;
+  int top;
+};
+
+/// Phase 1 of an insertion wave [s, e): for every node i of the wave,
+/// found[i - s][lc] = the ef_construction beam on layer lc (sorted by
+/// (distance, id)) for lc = min(level, frame.top) .. 0, searched on the frozen
+/// graph. A replacement (the device probe) must return exactly what the host's
+/// probe returns.
+using ProbeFn = std::function<void(const Graph& g, const WaveFrame& frame, uint32_t s, uint32_t e,
+                                   std::vector<std::vector<std::vector<Key>>>& found, Team& team)>;
+
 struct Graph {
   // parameters
   BuildParams params;
@@ -108,10 +129,6 @@ struct Graph {
   FileMap mapping;
   bool read_only = false;
 
-  // build-time synchronisation
-  std::mutex top_mutex;
-  std::unique_ptr<std::mutex[]> node_locks{new std::mutex[kNodeLocks]};
-
   explicit Graph(const BuildParams& p);
 
   const float* vec(uint32_t id) const noexcept { return vecs + static_cast<std::size_t>(id) * padded_dim; }
@@ -120,14 +137,20 @@ struct Graph {
                       : upper + upper_offset[id] + static_cast<std::size_t>(level - 1) * upper_stride;
   }
   std::size_t degree_cap(int level) const noexcept { return level == 0 ? M0 : M; }
-  std::mutex& lock_of(uint32_t id) const noexcept { return node_locks[id & (kNodeLocks - 1)]; }
-  float dist(const float* a, const float* b) const noexcept;
+  using DistFn = float (*)(const float*, const float*, std::size_t) noexcept;
+  DistFn dist_fn = nullptr;  // the metric's distance for this host's ISA
+  static DistFn pick_distance(int metric);
+  float dist(const float* a, const float* b) const noexcept { return dist_fn(a, b, padded_dim); }
 
   void reserve(std::size_t n);
   void rebind();
   /// add_batch: validates every label first, then appends (levels drawn in
-  /// input order) and links in parallel on `threads` workers (0 = all cores).
-  void add_batch(const float* vecs_in, const uint64_t* labels_in, std::size_t n, std::size_t threads);
+  /// input order) and links the new nodes in insertion waves on `threads`
+  /// workers (0 = all cores). `probe`, if set, replaces the host's phase 1.
+  void add_batch(const float* vecs_in, const uint64_t* labels_in, std::size_t n, std::size_t threads,
+                 const ProbeFn& probe = {});
+  /// Store one row, label and level (the sequential part of an insertion).
+  uint32_t append(const float* v, uint64_t label);
 
   void save(const std::string& path) const;
   static std::unique_ptr<Graph> load(const std::string& path, bool mmap);

@@ -72,6 +72,29 @@ def test_truncation_and_bit_flips_are_rejected(lib, tmp_path):
             lib.Index.load(bad, mmap=bool(pos & 1))
 
 
+def test_parallel_build_is_deterministic_and_keeps_recall(lib, oracle, tmp_path):
+    """Multi-threaded builds insert in waves against the graph frozen at each
+    wave's start: the file is the same for any thread count >= 2, and recall
+    at a given ef stays within 0.01 of the one-thread (reference-order) build."""
+    import workloads
+
+    X = workloads.mixture_128(8000, seed=11, dim=32)
+    Q = workloads.mixture_128(300, seed=1011, dim=32)
+    files = {}
+    for threads in (1, 2, 5, 8):
+        ix = lib.Index(32, "l2", M=8, ef_construction=64)
+        ix.add(X, num_threads=threads)
+        files[threads] = tmp_path / f"t{threads}.idx"
+        ix.save(files[threads])
+    assert files[2].read_bytes() == files[5].read_bytes() == files[8].read_bytes()
+    assert files[2].read_bytes() != files[1].read_bytes()
+    truth = oracle.brute_force(X, Q, 10)["label"].astype(np.int64)
+    for ef in (10, 20, 40):
+        r1 = workloads.recall(oracle.load(files[1]).search(Q, 10, ef)["label"].astype(np.int64), truth, 10)
+        r8 = workloads.recall(oracle.load(files[8]).search(Q, 10, ef)["label"].astype(np.int64), truth, 10)
+        assert r8 >= r1 - 0.01, (ef, r1, r8)
+
+
 def test_reference_loads_our_files(golden, lib, reference, tmp_path):
     X = golden["cos_d100/data"]
     ix = lib.Index(100, "cosine", M=16, ef_construction=100)
