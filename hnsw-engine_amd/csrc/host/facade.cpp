@@ -26,6 +26,7 @@
 #include "../../../include/hnswgpu.h"
 #include "../error.hpp"
 #include "../gpu/brute.hpp"
+#include "../gpu/build_probe.hpp"
 #include "distance.hpp"
 #include "graph.hpp"
 
@@ -96,6 +97,10 @@ std::size_t BitsetFilter::count() const noexcept {
 // Index
 // ---------------------------------------------------------------------------
 namespace {
+/// Waves smaller than this before the first device wave are probed on the
+/// host (too few queries to fill the GPU).
+constexpr uint32_t kMinDeviceWave = 512;
+
 hnswamd::BuildParams to_build(const Params& p) {
   hnswamd::BuildParams b;
   b.dim = p.dim;
@@ -231,6 +236,22 @@ void Index::add_batch(const float* vecs, const uint64_t* labels, std::size_t n, 
   std::unique_lock<std::shared_mutex> lk(impl_->api);
   impl_->refuse_if_served("add_batch");
   impl_->g->add_batch(vecs, labels, n, num_threads);
+  std::lock_guard<std::mutex> dk(impl_->dev_mu);
+  impl_->dev_stale = true;
+}
+
+void Index::add_batch_device(const float* vecs, const uint64_t* labels, std::size_t n, std::size_t num_threads,
+                             DeviceBuildStats* stats) {
+  if (n == 0) return;
+  if (vecs == nullptr || labels == nullptr) throw std::invalid_argument("add_batch: null input");
+  std::unique_lock<std::shared_mutex> lk(impl_->api);
+  impl_->refuse_if_served("add_batch");
+  hnswamd::DeviceProbeStats st;
+  {
+    // The probe's device session ends (and frees its memory) with the call.
+    impl_->g->add_batch(vecs, labels, n, num_threads, hnswamd::make_device_probe(kMinDeviceWave, &st));
+  }
+  if (stats != nullptr) *stats = DeviceBuildStats{st.waves, st.device_nodes, st.host_nodes, st.device_ms, st.refresh_ms};
   std::lock_guard<std::mutex> dk(impl_->dev_mu);
   impl_->dev_stale = true;
 }
@@ -516,6 +537,19 @@ int hnsw_index_save(hnsw_index* h, const char* path) {
 
 int hnsw_index_add_batch(hnsw_index* h, const float* vecs, const uint64_t* labels, uint64_t n, uint64_t threads) {
   return guarded([&] { h->idx.add_batch(vecs, labels, n, threads); });
+}
+
+int hnsw_index_add_batch_device(hnsw_index* h, const float* vecs, const uint64_t* labels, uint64_t n,
+                                uint64_t threads, hnswgpu_build_stats* stats) {
+  static_assert(sizeof(hnswgpu_build_stats) == sizeof(hnsw::Index::DeviceBuildStats), "build stats layout");
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("add_batch: null handle");
+    hnsw::Index::DeviceBuildStats st;
+    h->idx.add_batch_device(vecs, labels, n, threads, &st);
+    if (stats != nullptr) {
+      *stats = hnswgpu_build_stats{st.waves, st.device_nodes, st.host_nodes, st.device_ms, st.refresh_ms};
+    }
+  });
 }
 
 int hnsw_index_search_batch(hnsw_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,

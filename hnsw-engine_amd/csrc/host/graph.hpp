@@ -81,10 +81,13 @@ struct Key {
   uint32_t id;
 };
 
-/// The graph as it stood before an insertion wave: its entry point and top level.
+/// The graph as it stood before an insertion wave: its entry point and top
+/// level, and the ids whose lists the previous wave of the same add_batch
+/// changed (on any layer; empty before the first wave).
 struct WaveFrame {
   uint32_t entry;
   int top;
+  const std::vector<uint32_t>* changed;
 };
 
 /// Phase 1 of an insertion wave [s, e): for every node i of the wave,
@@ -146,7 +149,8 @@ struct Graph {
   void rebind();
   /// add_batch: validates every label first, then appends (levels drawn in
   /// input order) and links the new nodes in insertion waves on `threads`
-  /// workers (0 = all cores). `probe`, if set, replaces the host's phase 1.
+  /// workers (0 = all cores). `probe`, if set, replaces the host's phase 1
+  /// and must return exactly what the host's probe would.
   void add_batch(const float* vecs_in, const uint64_t* labels_in, std::size_t n, std::size_t threads,
                  const ProbeFn& probe = {});
   /// Store one row, label and level (the sequential part of an insertion).

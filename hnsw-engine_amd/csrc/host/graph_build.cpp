@@ -20,8 +20,8 @@
 // With one node per wave this is the reference's sequential insertion
 // (src/index.cpp:205-306): a one-thread build writes the reference's bytes.
 // Multi-threaded builds use waves that grow with the graph (1/32 of it, at most
-// 1024 nodes), so their graph depends on the input alone, not on the thread
-// count or on scheduling. Nodes of one wave do not see each other on layer 0
+// 16,384 nodes), so their graph depends on the input alone, not on the thread
+// count, on scheduling or on where phase 1 ran. Nodes of one wave do not see each other on layer 0
 // (at 1/32 of the graph that cost no recall: 100k x 128 mixture, recall@10 at
 // ef 16..128 equal to a one-thread build within 0.001); on the sparse upper
 // layers, and above the frozen top, they are joined as candidates.
@@ -307,11 +307,12 @@ void probe_on_host(const Graph& g, const WaveFrame& f, uint32_t id, std::vector<
 // ---------------------------------------------------------------------------
 std::size_t wave_length(std::size_t frozen, std::size_t threads) {
   if (threads <= 1 || frozen == 0) return 1;
-  return std::clamp<std::size_t>(frozen / 32, 1, 1024);
+  return std::clamp<std::size_t>(frozen / 32, 1, kMaxWave);
 }
 
-void insert_wave(Graph& g, uint32_t s, uint32_t e, Team& team, const ProbeFn& probe) {
-  const WaveFrame frame{g.entry_point, g.max_level};
+void insert_wave(Graph& g, uint32_t s, uint32_t e, Team& team, const ProbeFn& probe,
+                 std::vector<uint32_t>& changed) {
+  const WaveFrame frame{g.entry_point, g.max_level, &changed};
   const std::size_t n = e - s;
   std::vector<std::vector<std::vector<Key>>> found(n);
 
@@ -381,6 +382,18 @@ void insert_wave(Graph& g, uint32_t s, uint32_t e, Team& team, const ProbeFn& pr
       store_list(l, s_.kept);
     }
   });
+
+  // what the next wave's probe has to refresh: the new nodes and the targets
+  std::vector<uint32_t> next_changed;
+  next_changed.reserve(n + starts.size());
+  for (uint32_t id = s; id < e; ++id) next_changed.push_back(id);
+  for (std::size_t gi = 0; gi + 1 < starts.size(); ++gi) {
+    const uint32_t t = all[starts[gi]].target;
+    if (t < s || t >= e) next_changed.push_back(t);
+  }
+  std::sort(next_changed.begin(), next_changed.end());
+  next_changed.erase(std::unique(next_changed.begin(), next_changed.end()), next_changed.end());
+  changed.swap(next_changed);
 
   // the highest new level becomes the entry point (the first such node)
   for (uint32_t id = s; id < e; ++id) {
@@ -515,9 +528,10 @@ void Graph::add_batch(const float* vin, const uint64_t* lin, std::size_t n, std:
   for (std::size_t i = 0; i < n; ++i) append(vin + i * dim, lin[i]);
   const std::size_t nt = threads == 0 ? std::max(1u, std::thread::hardware_concurrency()) : threads;
   Team team(std::min(nt, n));
+  std::vector<uint32_t> changed;
   for (uint32_t s = first, end = first + static_cast<uint32_t>(n); s < end;) {
     const uint32_t e = std::min<uint32_t>(end, s + static_cast<uint32_t>(wave_length(s, nt)));
-    insert_wave(*this, s, e, team, probe);
+    insert_wave(*this, s, e, team, probe, changed);
     s = e;
   }
 }

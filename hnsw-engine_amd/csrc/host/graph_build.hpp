@@ -50,11 +50,16 @@ class Team {
 /// Phase 1 for one node on the host.
 void probe_on_host(const Graph& g, const WaveFrame& frame, uint32_t id, std::vector<std::vector<Key>>& by_layer);
 
+/// Largest insertion wave.
+inline constexpr std::size_t kMaxWave = 16384;
+
 /// Nodes per wave when `frozen` nodes are linked: 1 for one-thread builds
-/// (the reference's sequential order), else 1/32 of the graph, 1..1024.
+/// (the reference's sequential order), else 1/32 of the graph, 1..kMaxWave.
 std::size_t wave_length(std::size_t frozen, std::size_t threads);
 
-/// Insert ids [s, e), already appended, as one wave.
-void insert_wave(Graph& g, uint32_t s, uint32_t e, Team& team, const ProbeFn& probe);
+/// Insert ids [s, e), already appended, as one wave. `changed` holds the ids
+/// whose lists the previous wave changed and is replaced by this wave's.
+void insert_wave(Graph& g, uint32_t s, uint32_t e, Team& team, const ProbeFn& probe,
+                 std::vector<uint32_t>& changed);
 
 }  // namespace hnswamd

@@ -78,6 +78,7 @@ def _load():
         "hnsw_index_free": ([p], None),
         "hnsw_index_save": ([p, C.c_char_p], C.c_int),
         "hnsw_index_add_batch": ([p, p, p, u64, u64], C.c_int),
+        "hnsw_index_add_batch_device": ([p, p, p, u64, u64, p], C.c_int),
         "hnsw_index_search_batch": ([p, p, u64, u64, u64, p, u64, p, u64], C.c_int),
         "hnsw_index_search": ([p, p, u64, u64, p, u64, p, p], C.c_int),
         "hnsw_index_mark_deleted": ([p, u64], C.c_int),
@@ -280,6 +281,12 @@ class ServerConfig(C.Structure):
                 ("max_cost", C.c_uint64), ("heavy_cost", C.c_uint64)]
 
 
+class BuildStats(C.Structure):
+    """hnswgpu_build_stats: what a device-probed add did."""
+    _fields_ = [("waves", C.c_uint64), ("device_nodes", C.c_uint64), ("host_nodes", C.c_uint64),
+                ("device_ms", C.c_double), ("refresh_ms", C.c_double)]
+
+
 class Request(C.Structure):
     _fields_ = [("queries", C.c_void_p), ("nq", C.c_uint64), ("k", C.c_uint64), ("ef", C.c_uint64),
                 ("filter_words", C.c_void_p), ("filter_size", C.c_uint64), ("storage", C.c_int32),
@@ -480,7 +487,10 @@ class Index:
         _check(_lib.hnsw_index_info(self._h, _ptr(a)))
         return a
 
-    def add(self, vectors, ids=None, num_threads: int = 0) -> None:
+    def add(self, vectors, ids=None, num_threads: int = 0, device: bool = False):
+        """add_batch; device=True runs the insertion searches of level-0 nodes
+        on the GPU (the same graph as the host build with the same
+        num_threads) and returns its statistics as a dict."""
         v, _ = _f32(vectors, self.dim, "vectors")
         n = v.shape[0]
         if ids is None:
@@ -489,7 +499,12 @@ class Index:
             labels = np.ascontiguousarray(np.asarray(ids, dtype=np.uint64).reshape(-1))
             if labels.size != n:
                 raise ValueError("ids: length must match the number of vectors")
-        _check(_lib.hnsw_index_add_batch(self._h, _ptr(v), _ptr(labels), n, num_threads))
+        if not device:
+            _check(_lib.hnsw_index_add_batch(self._h, _ptr(v), _ptr(labels), n, num_threads))
+            return None
+        st = BuildStats()
+        _check(_lib.hnsw_index_add_batch_device(self._h, _ptr(v), _ptr(labels), n, num_threads, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in BuildStats._fields_}
 
     def search_rows(self, queries, k: int = 10, ef=None, filter=None):
         """Raw Neighbor rows (nq, k) with fields label/distance. filter: None,

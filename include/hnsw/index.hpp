@@ -49,6 +49,19 @@ class Index {
 
   void add(const float* vec, uint64_t label);
   void add_batch(const float* vecs, const uint64_t* labels, std::size_t n, std::size_t num_threads = 0);
+  /// What add_batch_device did: waves probed on the GPU, nodes whose layer-0
+  /// insertion search ran there or on the host, GPU time of those searches
+  /// and host time spent refreshing the device copy of the graph.
+  struct DeviceBuildStats {
+    uint64_t waves = 0, device_nodes = 0, host_nodes = 0;
+    double device_ms = 0.0, refresh_ms = 0.0;
+  };
+  /// add_batch with the insertion searches of level-0 nodes on the GPU (no
+  /// counterpart in the reference). The graph equals add_batch's with the
+  /// same num_threads (>= 2: insertion waves; 1: one node per wave, the
+  /// reference's sequential order).
+  void add_batch_device(const float* vecs, const uint64_t* labels, std::size_t n, std::size_t num_threads = 0,
+                        DeviceBuildStats* stats = nullptr);
 
   /// k nearest neighbours sorted by increasing distance; ef = 0 uses the index
   /// default, ef is clamped to >= k; returns min(k, #eligible) results.

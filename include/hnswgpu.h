@@ -359,6 +359,23 @@ int hnsw_index_save(hnsw_index* h, const char* path);
 /* Replaces: Index::add_batch / add (include/hnsw/index.hpp:47-50); CPU builder. */
 int hnsw_index_add_batch(hnsw_index* h, const float* vecs, const uint64_t* labels, uint64_t n,
                          uint64_t num_threads);
+/* What hnsw_index_add_batch_device did (hnsw::Index::DeviceBuildStats). */
+typedef struct hnswgpu_build_stats {
+  uint64_t waves;        /* insertion waves probed on the GPU */
+  uint64_t device_nodes; /* nodes whose layer-0 insertion search ran on the GPU */
+  uint64_t host_nodes;   /* nodes probed on the host (upper-level nodes, early small waves) */
+  double device_ms;      /* GPU time of those searches */
+  double refresh_ms;     /* host time validating and uploading the lists each wave changed */
+} hnswgpu_build_stats;
+
+/* No counterpart in the reference (its builder is CPU only): add_batch with the
+ * insertion searches of level-0 nodes on the GPU. The graph is the one
+ * hnsw_index_add_batch builds with the same num_threads: >= 2 inserts in waves
+ * against the graph frozen at each wave's start (1/32 of the graph, at most
+ * 16,384 nodes), 1 inserts one node at a time (the reference's order).
+ * stats may be NULL. */
+int hnsw_index_add_batch_device(hnsw_index* h, const float* vecs, const uint64_t* labels, uint64_t n,
+                                uint64_t num_threads, hnswgpu_build_stats* stats);
 /* Replaces: Index::search_batch, on the GPU (include/hnsw/index.hpp:59-60). */
 int hnsw_index_search_batch(hnsw_index* h, const float* qs, uint64_t nq, uint64_t k, uint64_t ef,
                             hnswgpu_neighbor* out, uint64_t num_threads, const uint64_t* filter_words,
