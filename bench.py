@@ -2,14 +2,16 @@
 
 Workload: the integer-valued Gaussian mixture (256 centres U[0,255]^128,
 sigma 12, clipped and rounded) at 1,000,000 base vectors, 10,000 queries from
-the same mixture; HNSW M=16, ef_construction=200, built on the host by the
-engine's builder (the reference's algorithm and file format); k=10.
+the same mixture; HNSW M=16, ef_construction=200, built by the engine's
+builder (the reference's algorithm and file format) with its insertion
+searches on the GPU (--host-build: on the host); k=10.
 
 A step is one search_batch of all queries (device buffers, inputs resident in
 HBM). For every ef of the sweep: `warmup` untimed steps, then `steps` timed
 steps bracketed by synchronisations and timed with HIP events on the stream the
-kernels run on. Recall is scored against exact ground truth (fp32 GEMM
-shortlist of 64 on the GPU, re-ranked in FP64 on the host). QPS at recall 0.95
+kernels run on. Recall is scored against exact ground truth from the engine's
+GPU brute force (exact for this data: every fp32 partial sum is an integer
+below 2^24), with 32 queries re-checked in FP64 on the host. QPS at recall 0.95
 is interpolated between the two bracketing ef points (log QPS linear in
 recall). The roofline figure uses the fast kernel's own event-timed duration
 and the algorithmic bytes per query counted by the kernel itself (rows,
@@ -17,7 +19,8 @@ layer-0 lists and upper lists read, query in, results out).
 
 The CPU baseline is the reference's search_batch (compiled from its sources
 into oracle/_ref) on the same index file with all host cores, over the same
-queries and ef sweep.
+queries and ef sweep; its rows are compared with the GPU's row by row. The
+reference's add_batch of the same data on the same cores is the build baseline.
 """
 
 from __future__ import annotations
@@ -224,8 +227,10 @@ def qps_at(points, target=0.95):
     return workloads.qps_at_recall(points, target)
 
 
-def cpu_baseline(index_path: str, Q: np.ndarray, truth: np.ndarray, k: int, efs, budget_s: float):
-    """The reference's search_batch on this host's cores (oracle/_ref)."""
+def cpu_baseline(index_path: str, Q: np.ndarray, truth: np.ndarray, k: int, efs, budget_s: float, X=None):
+    """The reference's search_batch on this host's cores (oracle/_ref), and,
+    when X is given, the reference's add_batch of X on the same cores (the
+    build baseline; the time includes writing the file)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     from oracle import REF_SO, Reference
 
@@ -233,6 +238,14 @@ def cpu_baseline(index_path: str, Q: np.ndarray, truth: np.ndarray, k: int, efs,
     if not REF_SO.exists():
         return None
     ref = Reference()
+    ref_build_s = None
+    if X is not None:
+        path = os.path.join(tempfile.gettempdir(), "hnsw_bench_reference_build.idx")
+        t = time.perf_counter()
+        ref.build_save(path, X, metric=0, M=16, efc=200, threads=cores)
+        ref_build_s = time.perf_counter() - t
+        os.remove(path)
+        log(f"reference add_batch of {X.shape[0]} x {X.shape[1]} on {cores} threads (+ save): {ref_build_s:.1f}s")
     idx = ref.load(index_path, mmap=True)
     # Bounded sample: size it from a short probe at the largest ef.
     probe = Q[:200]
@@ -252,7 +265,7 @@ def cpu_baseline(index_path: str, Q: np.ndarray, truth: np.ndarray, k: int, efs,
         points.append((ef, rec, nq / dt))
         log(f"cpu reference ef={ef} recall={rec:.4f} qps={nq / dt:,.0f} ({cores} threads)")
     return {"value": qps_at(points), "unit": "queries/s", "cores": cores, "kind": "reference",
-            "isa": ref.active_isa(),
+            "isa": ref.active_isa(), "reference_build_s": round(ref_build_s, 2) if ref_build_s else None,
             "sample": f"first {nq} of the {Q.shape[0]} queries, ef sweep {list(efs)}, search_batch on {cores} threads",
             "points": [{"ef": e, "recall": round(r, 5), "qps": round(q, 1)} for e, r, q in points],
             "_rows": rows_by_ef}
@@ -319,8 +332,9 @@ def reduced_precision_run(args, which: str) -> dict:
     threads = args.build_threads or len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
     ix = hnswgpu.Index(dim, metric, M=16, ef_construction=200)
-    ix.add(X, num_threads=threads)
-    log(f"built {n} x {dim} {metric} on {threads} threads in {time.perf_counter() - t0:.1f}s (max level {ix.max_level})")
+    ix.add(X, num_threads=threads, device=not args.host_build)
+    log(f"built {n} x {dim} {metric} ({'host' if args.host_build else 'device'} probe, {threads} host threads) "
+        f"in {time.perf_counter() - t0:.1f}s (max level {ix.max_level})")
     kmax = max(ks)
     bf = hnswgpu.BruteForceIndex(dim, metric)
     bf.add(X)
@@ -417,6 +431,8 @@ def main() -> int:
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the 1M x 100 cosine bf16 and 200k x 960 fp8 configurations")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
+    ap.add_argument("--host-build", action="store_true",
+                    help="build the index with the host probe only (default: insertion searches on the GPU)")
     ap.add_argument("--serving-seconds", type=float, default=3.0)
     ap.add_argument("--serving-slots", type=int, default=SERVING_SLOTS)
     ap.add_argument("--serving-max-cost", type=int, default=0, help="cost units per launch (0: the server's default)")
@@ -468,6 +484,7 @@ def main() -> int:
         return 0
 
     index_path = args.index or os.path.join(tempfile.gettempdir(), f"hnsw_bench_{args.n}.idx")
+    build = None
     if args.index and os.path.exists(index_path):
         ix = hnswgpu.Index.load(index_path)
         log(f"loaded {index_path}")
@@ -475,9 +492,13 @@ def main() -> int:
         threads = args.build_threads or len(os.sched_getaffinity(0))
         t0 = time.perf_counter()
         ix = hnswgpu.Index(128, "l2", M=16, ef_construction=200)
-        ix.add(X, num_threads=threads)
+        stats = ix.add(X, num_threads=threads, device=not args.host_build)
         build_s = time.perf_counter() - t0
-        log(f"built {args.n} x 128 on {threads} threads in {build_s:.1f}s (max level {ix.max_level})")
+        build = {"builder": "host" if args.host_build else "device", "seconds": round(build_s, 2),
+                 "host_threads": threads, "max_level": ix.max_level}
+        if stats:
+            build.update({k: (round(v, 1) if isinstance(v, float) else v) for k, v in stats.items()})
+        log(f"built {args.n} x 128 ({build['builder']} probe, {threads} host threads) in {build_s:.1f}s: {build}")
         ix.save(index_path)
 
     if args.serving_only:
@@ -617,7 +638,7 @@ def main() -> int:
 
     cpu, parity = None, None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(index_path, Q, truth, k, efs, args.cpu_budget_s)
+        cpu = cpu_baseline(index_path, Q, truth, k, efs, args.cpu_budget_s, X=X if build is not None else None)
     if cpu is not None:
         # Row parity with the reference's own search_batch on the same file and queries: a row
         # mismatches if any label or distance bit differs (the GPU computes in the AVX-512 order).
@@ -665,6 +686,7 @@ def main() -> int:
                      "bytes_per_query": op["bytes_per_query"]},
         "cpu_baseline": cpu,
         "parity": parity,
+        "build": build,
         "brute_force": brute,
         "autotune": tuned,
         "serving": serving,

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <memory>
 #include <stdexcept>
@@ -78,7 +79,7 @@ double ms_since(std::chrono::steady_clock::time_point t) {
 
 class Session {
  public:
-  Session(const Graph& g, uint32_t frozen, DeviceProbeStats* stats) : stats_(stats) {
+  Session(const Graph& g, uint32_t frozen, DeviceProbeStats* stats, Team& team) : stats_(stats) {
     if (g.padded_dim > 4096) throw std::invalid_argument("add_batch_device: padded_dim must be <= 4096");
     if (2 * g.M > gpu::kMaxDegree0) throw std::invalid_argument("add_batch_device: M must be <= 32");
     if (g.count >= 0xFFFFFFFFull || g.upper_used >= 0xFFFFFFFFull) {
@@ -92,8 +93,13 @@ class Session {
     const hipStream_t st = h_->stream;
     // Every list of the frozen graph is checked once; lists of nodes not yet
     // linked are empty.
-    h_->view.has_dups = 0;
-    for (uint32_t id = 0; id < n; ++id) check_lists(g, id, frozen);
+    const uint64_t chunk = 4096;
+    team.run((n + chunk - 1) / chunk, [&](std::size_t c, std::size_t) {
+      for (uint64_t id = c * chunk; id < std::min<uint64_t>(n, (c + 1) * chunk); ++id) {
+        check_lists(g, static_cast<uint32_t>(id), frozen);
+      }
+    });
+    h_->view.has_dups = dups_.load() ? 1 : 0;
     auto put = [&](DevBuf& b, const void* src, size_t bytes) {
       void* d = b.need(std::max<size_t>(bytes, 4));
       if (bytes > 0) check(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, st), "hipMemcpy upload");
@@ -133,7 +139,7 @@ class Session {
 
   /// Lists changed by the previous wave: checked, staged and scattered into
   /// the device layer-0 array; the (small) upper section is copied whole.
-  void refresh(const Graph& g, const std::vector<uint32_t>& changed, uint32_t frozen) {
+  void refresh(const Graph& g, const std::vector<uint32_t>& changed, uint32_t frozen, Team& team) {
     const auto t0 = std::chrono::steady_clock::now();
     const hipStream_t st = h_->stream;
     check(hipStreamSynchronize(st), "refresh");  // staging buffers are reused
@@ -141,14 +147,18 @@ class Session {
     if (m > 0) {
       auto* ids = static_cast<uint32_t*>(stage_changed_.need(m * sizeof(uint32_t)));
       auto* rows = static_cast<uint32_t*>(stage_rows_.need(m * g.l0_stride * sizeof(uint32_t)));
-      for (size_t i = 0; i < m; ++i) {
-        const uint32_t id = changed[i];
-        if (id >= h_->view.count) throw std::logic_error("add_batch_device: changed id out of range");
-        check_lists(g, id, frozen);
-        ids[i] = id;
-        std::copy(g.l0 + static_cast<size_t>(id) * g.l0_stride, g.l0 + static_cast<size_t>(id + 1) * g.l0_stride,
-                  rows + i * g.l0_stride);
-      }
+      const size_t chunk = 1024;
+      team.run((m + chunk - 1) / chunk, [&](std::size_t c, std::size_t) {
+        for (size_t i = c * chunk; i < std::min(m, (c + 1) * chunk); ++i) {
+          const uint32_t id = changed[i];
+          if (id >= h_->view.count) throw std::logic_error("add_batch_device: changed id out of range");
+          check_lists(g, id, frozen);
+          ids[i] = id;
+          std::copy(g.l0 + static_cast<size_t>(id) * g.l0_stride,
+                    g.l0 + static_cast<size_t>(id + 1) * g.l0_stride, rows + i * g.l0_stride);
+        }
+      });
+      if (dups_.load()) h_->view.has_dups = 1;
       auto* d_ids = static_cast<uint32_t*>(d_changed_.need(m * sizeof(uint32_t)));
       auto* d_rows = static_cast<uint32_t*>(d_rows_.need(m * g.l0_stride * sizeof(uint32_t)));
       check(hipMemcpyAsync(d_ids, ids, m * sizeof(uint32_t), hipMemcpyHostToDevice, st), "hipMemcpy ids");
@@ -261,7 +271,7 @@ class Session {
         uint32_t ids[gpu::kMaxDegree0];
         std::copy(l + 1, l + 1 + l[0], ids);
         std::sort(ids, ids + l[0]);
-        if (std::adjacent_find(ids, ids + l[0]) != ids + l[0]) h_->view.has_dups = 1;
+        if (std::adjacent_find(ids, ids + l[0]) != ids + l[0]) dups_.store(true);
       }
     }
   }
@@ -271,6 +281,7 @@ class Session {
   DevBuf ones_buf_, d_changed_, d_rows_, d_ids_, d_q_, d_out_;
   PinnedBuf stage_changed_, stage_rows_, stage_ids_, stage_out_;
   float* ones_ = nullptr;
+  std::atomic<bool> dups_{false};
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   DeviceProbeStats* stats_;
 };
@@ -290,9 +301,9 @@ ProbeFn make_device_probe(uint32_t min_device_wave, DeviceProbeStats* stats) {
       return;
     }
     if (!ses) {
-      ses = std::make_unique<Session>(g, s, stats);
+      ses = std::make_unique<Session>(g, s, stats, team);
     } else {
-      ses->refresh(g, *f.changed, s);
+      ses->refresh(g, *f.changed, s, team);
     }
     ses->probe(g, f, s, e, found, team);
   };
