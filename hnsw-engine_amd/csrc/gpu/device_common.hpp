@@ -118,6 +118,7 @@ struct WaveSorted {
   }
   /// Insert x keeping the first `cap` (<= 64 * E) elements. Returns the element
   /// that no longer fits (x itself, or the old element at cap - 1), or kKeyInf.
+  /// Rows wholly before x's position do not change and are skipped.
   __device__ __forceinline__ uint64_t insert(uint64_t x, uint32_t cap) {
     const uint32_t pos = rank(x);
     if (pos >= cap) return x;
@@ -125,19 +126,23 @@ struct WaveSorted {
     const int lane = lane_id();
 #pragma unroll
     for (int e = E - 1; e >= 0; --e) {
-      const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
-      // Lane 0 of row e takes the last element of row e - 1 (unused in row 0).
-      const uint64_t up = wave_shr1_64(v[e], e > 0 ? rdlane64(v[e > 0 ? e - 1 : 0], 63) : kKeyInf);
-      uint64_t nv = v[e];
-      if (k == pos) nv = x;
-      if (k > pos) nv = up;
-      if (k >= cap) nv = kKeyInf;
-      v[e] = nv;
+      if (static_cast<uint32_t>(e) >= (pos >> 6)) {
+        const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
+        // Lane 0 of row e takes the last element of row e - 1 (unused in row 0).
+        const uint64_t up = wave_shr1_64(v[e], e > 0 ? rdlane64(v[e > 0 ? e - 1 : 0], 63) : kKeyInf);
+        uint64_t nv = v[e];
+        if (k == pos) nv = x;
+        if (k > pos) nv = up;
+        if (k >= cap) nv = kKeyInf;
+        v[e] = nv;
+      }
     }
     return out;
   }
   /// insert() that also moves a per-slot flag word with its entry; x gets
-  /// `xflag`, and the flag of the element pushed out is stored in out_flag.
+  /// `xflag`, slots at cap and beyond get flag 1 (the flag of an empty slot),
+  /// and the flag of the element pushed out is stored in out_flag (0 when x
+  /// itself does not fit).
   __device__ __forceinline__ uint64_t insert_flagged(uint64_t x, uint32_t cap, uint32_t (&fl)[E], uint32_t xflag,
                                                      uint32_t& out_flag) {
     const uint32_t pos = rank(x);
@@ -151,27 +156,29 @@ struct WaveSorted {
     const int lane = lane_id();
 #pragma unroll
     for (int e = E - 1; e >= 0; --e) {
-      const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
-      // Lane 0 of row e takes the last entry of row e - 1 (unused in row 0).
-      const int pe = e > 0 ? e - 1 : 0;
-      const uint64_t up = wave_shr1_64(v[e], e > 0 ? rdlane64(v[pe], 63) : kKeyInf);
-      const uint32_t fup = wave_shr1(fl[e], e > 0 ? rdlane(fl[pe], 63) : 0u);
-      uint64_t nv = v[e];
-      uint32_t nf = fl[e];
-      if (k == pos) {
-        nv = x;
-        nf = xflag;
+      if (static_cast<uint32_t>(e) >= (pos >> 6)) {
+        const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
+        // Lane 0 of row e takes the last entry of row e - 1 (unused in row 0).
+        const int pe = e > 0 ? e - 1 : 0;
+        const uint64_t up = wave_shr1_64(v[e], e > 0 ? rdlane64(v[pe], 63) : kKeyInf);
+        const uint32_t fup = wave_shr1(fl[e], e > 0 ? rdlane(fl[pe], 63) : 1u);
+        uint64_t nv = v[e];
+        uint32_t nf = fl[e];
+        if (k == pos) {
+          nv = x;
+          nf = xflag;
+        }
+        if (k > pos) {
+          nv = up;
+          nf = fup;
+        }
+        if (k >= cap) {
+          nv = kKeyInf;
+          nf = 1u;
+        }
+        v[e] = nv;
+        fl[e] = nf;
       }
-      if (k > pos) {
-        nv = up;
-        nf = fup;
-      }
-      if (k >= cap) {
-        nv = kKeyInf;
-        nf = 0;
-      }
-      v[e] = nv;
-      fl[e] = nf;
     }
     return out;
   }

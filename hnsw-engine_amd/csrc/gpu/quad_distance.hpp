@@ -146,5 +146,60 @@ __device__ __forceinline__ float quad_raw_rows(const float* __restrict__ row, co
   return active ? r : 0.0f;
 }
 
+/// The loads of one fp32 row of PC whole chunks by one quad lane (lane g:
+/// positions 4g..4g+3 of every chunk), issued without waiting, so the loads
+/// of several rows can be in flight together; sum() is quad_raw_rows' sum of
+/// them (the same summation order, bit for bit).
+template <int PC>
+struct F32RowRegs {
+  float4 x[PC];
+
+  __device__ __forceinline__ void load(const float* __restrict__ row) {
+    const uint32_t g4 = static_cast<uint32_t>(lane_id() & 3) * 4;
+#pragma unroll
+    for (int j = 0; j < PC; ++j) x[j] = *reinterpret_cast<const float4*>(row + 16 * j + g4);
+  }
+  template <bool L2>
+  __device__ __forceinline__ float sum(const float* q) const {
+    constexpr int kFull = (PC >> 2) << 2;
+    const uint32_t g4 = static_cast<uint32_t>(lane_id() & 3) * 4;
+    float a[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) a[i][e] = 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < PC; ++j) {
+      if (j > 0 && (j & 3) == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) asm volatile("" : "+v"(a[i][e])::"memory");
+        }
+      }
+      const float4 y = *reinterpret_cast<const float4*>(q + 16 * j + g4);
+      if (j < kFull) {
+        switch (j & 3) {
+          case 0: acc4(a[0], y, x[j], L2); break;
+          case 1: acc4(a[1], y, x[j], L2); break;
+          case 2: acc4(a[2], y, x[j], L2); break;
+          default: acc4(a[3], y, x[j], L2); break;
+        }
+      } else {
+        acc4(a[0], y, x[j], L2);
+      }
+    }
+    float v[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = (a[0][e] + a[1][e]) + (a[2][e] + a[3][e]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = v[e] + quad_xor2(v[e]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = v[e] + quad_xor1(v[e]);
+    return (v[0] + v[2]) + (v[1] + v[3]);
+  }
+};
+
 }  // namespace gpu
 }  // namespace hnswamd

@@ -153,5 +153,64 @@ __device__ __forceinline__ float quad_dot_fp8(const uint8_t* __restrict__ row, c
   return active ? v : 0.0f;
 }
 
+/// The loads of one bf16 row of PC 16-element chunks (PC / 2 16-byte pieces
+/// per quad lane) issued without waiting; dot() is quad_dot_bf16's sum of
+/// them (the same order, bit for bit).
+template <int PC>
+struct BF16RowRegs {
+  static_assert(PC % 2 == 0, "bf16 rows of whole 16-byte pieces per lane");
+  uint4 x[PC / 2];
+
+  __device__ __forceinline__ void load(const uint16_t* __restrict__ row) {
+    const uint32_t g = static_cast<uint32_t>(lane_id() & 3);
+    const uint4* r4 = reinterpret_cast<const uint4*>(row);
+#pragma unroll
+    for (int i = 0; i < PC / 2; ++i) x[i] = r4[g + 4 * i];
+  }
+  __device__ __forceinline__ float dot(const uint16_t* q) const {
+    const uint32_t g = static_cast<uint32_t>(lane_id() & 3);
+    const uint4* q4 = reinterpret_cast<const uint4*>(q);
+    float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < PC / 2; ++i) {
+      const uint4 y = q4[g + 4 * i];
+      a0 = dot2_bf16(x[i].x, y.x, a0);
+      a1 = dot2_bf16(x[i].y, y.y, a1);
+      a0 = dot2_bf16(x[i].z, y.z, a0);
+      a1 = dot2_bf16(x[i].w, y.w, a1);
+    }
+    float v = a0 + a1;
+    v = v + quad_xor1(v);
+    v = v + quad_xor2(v);
+    return v;
+  }
+};
+
+/// The loads of one e4m3 row of PC 16-element chunks (PC / 4 pieces per quad
+/// lane) issued without waiting; dot() is quad_dot_fp8's sum of them (before
+/// the row's scale).
+template <int PC>
+struct FP8RowRegs {
+  static_assert(PC % 4 == 0, "fp8 rows of whole 16-byte pieces per lane");
+  uint4 x[PC / 4];
+
+  __device__ __forceinline__ void load(const uint8_t* __restrict__ row) {
+    const uint32_t g = static_cast<uint32_t>(lane_id() & 3);
+    const uint4* r4 = reinterpret_cast<const uint4*>(row);
+#pragma unroll
+    for (int i = 0; i < PC / 4; ++i) x[i] = r4[g + 4 * i];
+  }
+  __device__ __forceinline__ float dot(const float* q) const {
+    const uint32_t g = static_cast<uint32_t>(lane_id() & 3);
+    float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < PC / 4; ++i) fp8_piece(x[i], q + 16 * (g + 4 * i), a0, a1);
+    float v = a0 + a1;
+    v = v + quad_xor1(v);
+    v = v + quad_xor2(v);
+    return v;
+  }
+};
+
 }  // namespace gpu
 }  // namespace hnswamd

@@ -105,14 +105,91 @@ __device__ __forceinline__ WaveLds wave_lds(uint32_t* base, uint32_t log2h, uint
                  reinterpret_cast<float*>(base + H + 192 + padded + padded / 2)};
 }
 
+/// The loads one quad lane makes for one row of a fixed width (PC chunks) of
+/// storage ST, and the row's distance from them: the row itself, and for a
+/// reduced copy its |x^|^2 (L2) and fp8 scale. Inactive quads load row 0.
+template <int PC, int ST>
+struct RowRegs;
+
+template <int PC>
+struct RowRegs<PC, kStoreF32> {
+  F32RowRegs<PC> r;
+  template <bool L2>
+  __device__ __forceinline__ void load(const DevIndexView& ix, uint32_t id, bool) {
+    r.load(ix.vecs + static_cast<size_t>(id) * (16 * PC));
+  }
+  template <bool L2>
+  __device__ __forceinline__ float dist(const WaveLds& w, float) const {
+    const float d = r.template sum<L2>(w.q);
+    return L2 ? d : 1.0f - d;
+  }
+};
+
+template <int PC>
+struct RowRegs<PC, kStoreBF16> {
+  BF16RowRegs<PC> r;
+  float xn;
+  template <bool L2>
+  __device__ __forceinline__ void load(const DevIndexView& ix, uint32_t id, bool active) {
+    xn = (L2 && active) ? ix.vnorm[id] : 0.0f;
+    r.load(static_cast<const uint16_t*>(ix.vlow) + static_cast<size_t>(id) * (16 * PC));
+  }
+  template <bool L2>
+  __device__ __forceinline__ float dist(const WaveLds& w, float qn) const {
+    const float dp = r.dot(w.qb);
+    return L2 ? (qn + xn) - 2.0f * dp : 1.0f - dp;
+  }
+};
+
+template <int PC>
+struct RowRegs<PC, kStoreFP8> {
+  FP8RowRegs<PC> r;
+  float xn, xs;
+  template <bool L2>
+  __device__ __forceinline__ void load(const DevIndexView& ix, uint32_t id, bool active) {
+    xn = (L2 && active) ? ix.vnorm[id] : 0.0f;
+    xs = active ? ix.vscale[id] : 0.0f;
+    r.load(static_cast<const uint8_t*>(ix.vlow) + static_cast<size_t>(id) * (16 * PC));
+  }
+  template <bool L2>
+  __device__ __forceinline__ float dist(const WaveLds& w, float qn) const {
+    const float dp = r.dot(w.q) * xs;
+    return L2 ? (qn + xn) - 2.0f * dp : 1.0f - dp;
+  }
+};
+
 /// Ordered distances of the rows w.nid[0..U) into w.nd[0..U), 16 rows per step,
 /// from the fp32 rows (ST = 0, exact) or the bf16 / fp8 copy (ST = 1 / 2).
 /// Every load a row needs (its norm and scale included) is issued before its
-/// arithmetic, so a step costs one memory round trip.
-template <int PC, int ST, bool L2, int G, int BG>
+/// arithmetic, so a step costs one memory round trip. With a fixed row width
+/// (PC > 0), RT steps (16 * RT rows) share one round trip: all their loads
+/// are issued before the arithmetic of the first.
+template <int PC, int ST, bool L2, int G, int BG, int RT>
 __device__ __forceinline__ void row_dists_metric(const DevIndexView& ix, const WaveLds& w, uint32_t U) {
   const int lane = lane_id();
   const float qn = ST == kStoreF32 ? 0.0f : w.qn[0];
+  if constexpr (PC > 0) {
+    for (uint32_t base = 0; base < U; base += 16 * RT) {
+      RowRegs<PC, ST> rr[RT];
+#pragma unroll
+      for (int s = 0; s < RT; ++s) {
+        if (s == 0 || base + 16u * s < U) {
+          const uint32_t r = base + 16u * s + static_cast<uint32_t>(lane >> 2);
+          const bool active = r < U;
+          rr[s].template load<L2>(ix, active ? w.nid[r] : 0u, active);
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < RT; ++s) {
+        if (s == 0 || base + 16u * s < U) {
+          const uint32_t r = base + 16u * s + static_cast<uint32_t>(lane >> 2);
+          const float d = rr[s].template dist<L2>(w, qn);
+          if (r < U && (lane & 3) == 0) w.nd[r] = ord_of(d);
+        }
+      }
+    }
+    return;
+  }
   for (uint32_t base = 0; base < U; base += 16) {
     const uint32_t r = base + static_cast<uint32_t>(lane >> 2);
     const bool active = r < U;
@@ -142,14 +219,24 @@ __device__ __forceinline__ void row_dists_metric(const DevIndexView& ix, const W
 /// G / BG: group sizes of the run-time-width fp32 / bf16 row loops. The
 /// largest capacity class (ER = 8) takes smaller groups: its registers would
 /// otherwise spill to scratch.
-template <int PC, int ST = kStoreF32, int G = 8, int BG = 4>
+/// RT: 16-row steps per memory round trip where the row width is fixed.
+template <int PC, int ST = kStoreF32, int G = 8, int BG = 4, int RT = 1>
 __device__ __forceinline__ void row_dists(const DevIndexView& ix, const WaveLds& w, uint32_t U) {
   if (ix.metric == 0) {
-    row_dists_metric<PC, ST, true, G, BG>(ix, w, U);
+    row_dists_metric<PC, ST, true, G, BG, RT>(ix, w, U);
   } else {
-    row_dists_metric<PC, ST, false, G, BG>(ix, w, U);
+    row_dists_metric<PC, ST, false, G, BG, RT>(ix, w, U);
   }
 }
+
+#ifndef HNSWGPU_F32_ROW_TRIP
+#define HNSWGPU_F32_ROW_TRIP 1
+#endif
+/// 16-row steps per round trip in the layer-0 expansion: a reduced copy's rows
+/// take half (bf16) or a quarter (fp8) of the registers of fp32 ones, so two
+/// steps share one trip there at no cost in occupancy.
+template <int ST>
+constexpr int kRowTrip = ST == kStoreF32 ? HNSWGPU_F32_ROW_TRIP : 2;
 
 template <int ctrl, int row_mask>
 __device__ __forceinline__ uint64_t dpp_min64(uint64_t v) {
@@ -199,43 +286,46 @@ __device__ __forceinline__ bool hash_has(const uint32_t* tab, uint32_t log2h, ui
   }
 }
 
-/// Compact LDS hash: 2^L slots of 16 bits, two per LDS word (0xFFFF empty).
+/// Compact LDS hash: 2^L slots of 16 bits, two per LDS word (0 = empty).
 /// Ids are permuted by an odd multiplier mod 2^B (a bijection on B-bit ids);
 /// the top L bits of the permuted id give the home slot and the other
-/// S = B - L <= 8 bits its remainder. An entry stores the remainder with the
-/// low 7 bits of the home slot, so an id's entry is the same in every slot it
-/// may occupy (linear probing, at most `maxp` <= 127 slots past home): within
-/// that window the low home bits identify the home slot, so the entry
-/// identifies the id exactly, and never equals 0xFFFF. Probes read the aligned
-/// group of 8 slots (16 bytes) holding the current slot and test all 8 against
-/// the one expected entry with whole-word bit operations: most lookups cost one
-/// LDS read and a few instructions (and one compare-and-swap to insert). The
-/// table must be 16-byte aligned, L >= 8.
+/// S = B - L <= 8 bits its remainder. An entry stores 1 + (the remainder with
+/// the low 7 bits of the home slot), so an id's entry is the same in every slot
+/// it may occupy (linear probing, at most `maxp` <= 127 slots past home):
+/// within that window the low home bits identify the home slot, so the entry
+/// identifies the id exactly, and it is never 0. Probes read the aligned group
+/// of 8 slots (16 bytes) holding the current slot and test all 8 against the
+/// one expected entry with packed 16-bit operations: most lookups cost one LDS
+/// read and a few instructions (and one compare-and-swap to insert). The table
+/// must be 16-byte aligned, L >= 8.
 struct Hash16 {
   uint32_t L, B, maxp;
 
-  __device__ __forceinline__ void place(uint32_t id, uint32_t& home, uint32_t& key) const {
+  /// Home slot and stored entry of id.
+  __device__ __forceinline__ void place(uint32_t id, uint32_t& home, uint32_t& entry) const {
     const uint32_t x = (id * 0x9E3779B1u) & ((1u << B) - 1u);
     const uint32_t S = B - L;
     home = x >> S;
-    key = ((home & 0x7Fu) << S) | (x & ((1u << S) - 1u));
+    entry = (((home & 0x7Fu) << S) | (x & ((1u << S) - 1u))) + 1u;
   }
-  /// Bit j: slot j of the group holds `key` or is empty (0xFFFF). key2 = key
-  /// in both halves. Bit tricks on whole words (no per-slot compares): a half
-  /// h is non-zero iff bit 15 of ((h & 0x7FFF) + 0x7FFF) | h is set.
-  static __device__ __forceinline__ uint32_t stop_mask(const u32x4 g, uint32_t key2) {
+  /// min(a, b) of each 16-bit half (v_pk_min_u16). Written out because the
+  /// compiler turns a packed minimum against a constant into per-half compares.
+  static __device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_pk_min_u16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+  }
+  /// Bit j set: slot j of the group holds the entry or is empty (a stop).
+  /// entry2 = the entry in both halves. A half of min(w ^ entry2, w) is zero
+  /// iff the slot holds the entry or is empty; min of that with 1 is 0 for a
+  /// stop and 1 otherwise, which is gathered into an 8-bit mask in slot order
+  /// (slot 2i: bit 0 of word i, slot 2i + 1: bit 16).
+  static __device__ __forceinline__ uint32_t stop_mask(const u32x4 g, uint32_t entry2) {
     const uint32_t wd[4] = {g.x, g.y, g.z, g.w};
-    uint32_t ns[4];
+    uint32_t z[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const uint32_t m = wd[i] ^ key2;   // zero half: the entry is key
-      const uint32_t e = ~wd[i];         // zero half: the slot is empty
-      const uint32_t nzm = ((m & 0x7FFF7FFFu) + 0x7FFF7FFFu) | m;
-      const uint32_t nze = ((e & 0x7FFF7FFFu) + 0x7FFF7FFFu) | e;
-      ns[i] = nzm & nze & 0x80008000u;   // bits 15 / 31 set: not a stop
-    }
-    // Slot 2i's flag to bit 2i, slot 2i + 1's to bit 16 + 2i, then the odd ones to 2i + 1.
-    const uint32_t x = (ns[0] >> 15) | (ns[1] >> 13) | (ns[2] >> 11) | (ns[3] >> 9);
+    for (int i = 0; i < 4; ++i) z[i] = pk_min_u16(pk_min_u16(wd[i] ^ entry2, wd[i]), 0x00010001u);
+    const uint32_t x = z[0] | (z[1] << 2) | (z[2] << 4) | (z[3] << 6);  // bits 2i and 16 + 2i
     return ~(x | (x >> 15)) & 0xFFu;
   }
   /// Slots of the group that lie in the probe window: j >= o (the current
@@ -248,22 +338,22 @@ struct Hash16 {
   }
   /// 1: inserted, 0: present, 2: no free slot within maxp probes.
   __device__ __forceinline__ int visit(uint32_t* words, uint32_t id) const {
-    uint32_t home, key;
-    place(id, home, key);
-    const uint32_t key2 = key | (key << 16);
+    uint32_t home, entry;
+    place(id, home, entry);
+    const uint32_t entry2 = entry | (entry << 16);
     const uint32_t smask = (1u << L) - 1u;
     uint32_t p = 0;  // probe distance of the first slot to examine
     for (;;) {
       const uint32_t slot = (home + p) & smask;
       const uint32_t o = slot & 7u, gi = slot >> 3;
       const u32x4 g = lds_read16(words + 4 * gi);
-      const uint32_t cand = stop_mask(g, key2) & window(o, p);
+      const uint32_t cand = stop_mask(g, entry2) & window(o, p);
       if (cand != 0u) {
         const uint32_t j = static_cast<uint32_t>(__builtin_ctz(cand));
         const uint32_t wi = j >> 1, sh = (j & 1u) * 16u;
         const uint32_t cur = wi == 0 ? g.x : (wi == 1 ? g.y : (wi == 2 ? g.z : g.w));
-        if (((cur >> sh) & 0xFFFFu) == key) return 0;
-        const uint32_t old = atomicCAS(words + 4 * gi + wi, cur, (cur & ~(0xFFFFu << sh)) | (key << sh));
+        if (((cur >> sh) & 0xFFFFu) == entry) return 0;
+        const uint32_t old = atomicCAS(words + 4 * gi + wi, cur, cur | (entry << sh));  // the half is 0
         if (old == cur) return 1;
         p = p - o + j;  // the word changed under us: look at this slot again
         continue;
@@ -273,21 +363,21 @@ struct Hash16 {
     }
   }
   __device__ __forceinline__ bool has(const uint32_t* words, uint32_t id) const {
-    uint32_t home, key;
-    place(id, home, key);
-    const uint32_t key2 = key | (key << 16);
+    uint32_t home, entry;
+    place(id, home, entry);
+    const uint32_t entry2 = entry | (entry << 16);
     const uint32_t smask = (1u << L) - 1u;
     uint32_t p = 0;
     for (;;) {
       const uint32_t slot = (home + p) & smask;
       const uint32_t o = slot & 7u, gi = slot >> 3;
       const u32x4 g = lds_read16(words + 4 * gi);
-      const uint32_t cand = stop_mask(g, key2) & window(o, p);
+      const uint32_t cand = stop_mask(g, entry2) & window(o, p);
       if (cand != 0u) {
         const uint32_t j = static_cast<uint32_t>(__builtin_ctz(cand));
         const uint32_t wi = j >> 1, sh = (j & 1u) * 16u;
         const uint32_t cur = wi == 0 ? g.x : (wi == 1 ? g.y : (wi == 2 ? g.z : g.w));
-        return ((cur >> sh) & 0xFFFFu) == key;
+        return ((cur >> sh) & 0xFFFFu) == entry;
       }
       p += 8u - o;
       if (p > maxp) return false;
@@ -314,6 +404,11 @@ struct Visited {
   bool lost;      // per lane: ... and there is no HBM table
 
   __device__ __forceinline__ uint32_t lds_slots() const { return 1u << (compact ? lds_log2 + 1 : lds_log2); }
+  /// Empty the LDS table (whole wave): 0 in compact slots, kEmptySlot in 32-bit ones.
+  __device__ __forceinline__ void clear_lds() {
+    const uint32_t e = compact ? 0u : kEmptySlot;
+    for (uint32_t i = static_cast<uint32_t>(lane_id()); i < (1u << lds_log2); i += 64) lds[i] = e;
+  }
   /// Room for `more` new ids? Switches to the HBM table when the LDS one is
   /// full; false when both are.
   __device__ __forceinline__ bool reserve(uint32_t more) {
@@ -616,17 +711,40 @@ __device__ void write_results(const DevIndexView& ix, const QueryArgs& a, const 
   }
 }
 
-/// Index of the first result slot not yet expanded, or ~0 if none.
+/// The result slots not yet expanded, as one ballot mask per register row:
+/// bit l of m[e] is slot 64e + l. Empty slots carry flag 1, so they never
+/// appear. Computed once per expansion; the pop and the peek both read it.
 template <int ER>
-__device__ __forceinline__ uint32_t first_unexpanded(const uint32_t (&fl)[ER], uint32_t sR) {
-  uint32_t slot = ~0u;
+struct Unexpanded {
+  uint64_t m[ER];
+
+  __device__ __forceinline__ void load(const uint32_t (&fl)[ER]) {
 #pragma unroll
-  for (int e = ER - 1; e >= 0; --e) {
-    const uint64_t m = ballot(static_cast<uint32_t>(e * 64 + lane_id()) < sR && fl[e] == 0u);
-    if (m != 0) slot = static_cast<uint32_t>(e * 64 + __builtin_ctzll(m));
+    for (int e = 0; e < ER; ++e) m[e] = ballot(fl[e] == 0u);
   }
-  return slot;
-}
+  /// Index of the first unexpanded slot, or ~0 if none.
+  __device__ __forceinline__ uint32_t first() const {
+    uint32_t slot = ~0u;
+#pragma unroll
+    for (int e = ER - 1; e >= 0; --e) {
+      if (m[e] != 0) slot = static_cast<uint32_t>(e * 64 + __builtin_ctzll(m[e]));
+    }
+    return slot;
+  }
+  /// The first unexpanded slot after `slot`, which must be first() (the
+  /// lowest bit of its row, with no bit in the rows before). Read-only, so the
+  /// masks stay in scalar registers.
+  __device__ __forceinline__ uint32_t after(uint32_t slot) const {
+    uint32_t r = ~0u;
+#pragma unroll
+    for (int e = ER - 1; e >= 0; --e) {
+      uint64_t mm = m[e];
+      if (static_cast<uint32_t>(e) == (slot >> 6)) mm &= mm - 1;
+      if (mm != 0) r = static_cast<uint32_t>(e * 64 + __builtin_ctzll(mm));
+    }
+    return r;
+  }
+};
 
 /// Admission of a whole expansion at once while R is full (|R| = ef): R
 /// becomes the ef smallest keys of R and of the candidates (lanes of `smask`,
@@ -643,13 +761,16 @@ __device__ bool admit_merged(WaveSorted<ER>& R, uint32_t (&fl)[ER], uint32_t ef,
   const bool in_s = ((smask >> lane) & 1u) != 0;
   // Per candidate j (uniform in its iteration): merged position p_j = keys
   // below it in R and among the candidates. Output slot k holds the candidate
-  // with p_j = k, else R[k - (candidates placed before k)].
-  uint32_t before[ER];
+  // with p_j = k, else R[k - (candidates placed before k)]. Candidates placed
+  // in rows before row e count for every lane of row e (a uniform count per
+  // row); only the row holding p_j needs per-lane work.
+  uint32_t before[ER], rows_before[ER];
   uint64_t nv[ER];
   bool from_s[ER];
 #pragma unroll
   for (int e = 0; e < ER; ++e) {
     before[e] = 0;
+    rows_before[e] = 0;
     nv[e] = kKeyInf;
     from_s[e] = false;
   }
@@ -661,16 +782,22 @@ __device__ bool admit_merged(WaveSorted<ER>& R, uint32_t (&fl)[ER], uint32_t ef,
     const uint32_t pj = R.rank(sj) + static_cast<uint32_t>(__popcll(ballot(in_s && skey < sj)));
     s_before_ef += pj < ef ? 1u : 0u;
     if (pj == ef) s_at_ef = sj;
+    const uint32_t pr = pj >> 6, pl = pj & 63u;
 #pragma unroll
     for (int e = 0; e < ER; ++e) {
-      const uint32_t k = static_cast<uint32_t>(e * 64 + lane);
-      before[e] += pj < k ? 1u : 0u;
-      if (pj == k) {
-        from_s[e] = true;
-        nv[e] = sj;
+      if (static_cast<uint32_t>(e) > pr) {
+        ++rows_before[e];
+      } else if (static_cast<uint32_t>(e) == pr) {
+        before[e] += static_cast<uint32_t>(lane) > pl ? 1u : 0u;
+        if (static_cast<uint32_t>(lane) == pl) {
+          from_s[e] = true;
+          nv[e] = sj;
+        }
       }
     }
   }
+#pragma unroll
+  for (int e = 0; e < ER; ++e) before[e] += rows_before[e];
   // The key after the last one kept (merged position ef) must be beyond the new worst distance.
   const uint64_t first_out = s_at_ef != kKeyInf ? s_at_ef : R.at(ef - s_before_ef);
   uint32_t nf[ER];
@@ -694,7 +821,7 @@ __device__ bool admit_merged(WaveSorted<ER>& R, uint32_t (&fl)[ER], uint32_t ef,
     if (!from_s[e]) nv[e] = rv;
     if (k >= ef) {
       nv[e] = kKeyInf;
-      nf[e] = 0u;
+      nf[e] = 1u;
     }
   }
   uint64_t worst = kKeyInf;
@@ -730,13 +857,13 @@ __device__ bool admit_merged(WaveSorted<ER>& R, uint32_t (&fl)[ER], uint32_t ef,
 /// result of a full R is dead (never expanded); D's head being dead ends the
 /// search, as Alg. 2's stop rule does. A live key pushed out of a full D hands
 /// the query to the general kernel.
-template <int ER, int PC, int ST, bool DEL = false>
+template <int ER, int PC, int ST, bool DEL = false, int RT = 1>
 __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s,
                                const WaveLds& w, Visited& vis, uint32_t qi) {
   const int lane = lane_id();
   const uint32_t ef = a.ef;
 
-  for (uint32_t i = static_cast<uint32_t>(lane); i < (1u << vis.lds_log2); i += 64) w.hash[i] = kEmptySlot;
+  vis.clear_lds();
   vis.in_lds = 0;
   vis.in_hbm = 0;
   Work work;
@@ -744,9 +871,9 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
 
   WaveSorted<ER> R;
   R.clear();
-  uint32_t fl[ER];
+  uint32_t fl[ER];  // 0: an entry not yet expanded; 1: expanded, or an empty slot
 #pragma unroll
-  for (int e = 0; e < ER; ++e) fl[e] = 0u;
+  for (int e = 0; e < ER; ++e) fl[e] = 1u;
   uint32_t ties = 0u, n_ties = 0u;
   WaveSorted<1> D;
   D.clear();
@@ -768,7 +895,10 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
   Handoff result = Handoff::kNone;
   for (;;) {
     uint64_t c;
-    const uint32_t slot = first_unexpanded<ER>(fl, sR);
+    Unexpanded<ER> ux;
+    ux.load(fl);
+    const uint32_t slot = ux.first();
+    bool popped = false;  // c is R's slot `slot`
     const uint64_t dmin = DEL ? rdlane64(D.v[0], 0) : kKeyInf;
     if (DEL && dmin < (slot != ~0u ? R.at(slot) : n_ties > 0 ? key_of(tau, rdlane(ties, 0)) : kKeyInf)) {
       c = dmin;
@@ -780,6 +910,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
       for (int e = 0; e < ER; ++e) {
         if (static_cast<uint32_t>(e) == (slot >> 6) && static_cast<uint32_t>(lane) == (slot & 63)) fl[e] = 1u;
       }
+      popped = true;
     } else if (n_ties > 0) {
       c = key_of(tau, rdlane(ties, 0));
       ties = wave_shl1(ties, rdlane(ties, 63));
@@ -793,7 +924,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
     }
     const uint32_t cid = key_id(c);
     const ListRegs cur = uniform(next.id) == cid ? next : load_list0(ix, cid);
-    const uint32_t peek = first_unexpanded<ER>(fl, sR);
+    const uint32_t peek = popped ? ux.after(slot) : slot;
     uint64_t peek_key = peek != ~0u ? R.at(peek) : kKeyInf;
     if constexpr (DEL) {
       const uint64_t d0 = rdlane64(D.v[0], 0);
@@ -810,7 +941,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
     vis.count(U);
     work.dist += U;
     ++work.exp0;
-    row_dists<PC, ST, 8, (ER >= 8 ? 2 : 4)>(ix, w, U);
+    row_dists<PC, ST, 8, (ER >= 8 ? 2 : 4), RT>(ix, w, U);
     wave_sync();
     const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
     const uint32_t id_u = w.nid[lane];
@@ -1002,14 +1133,14 @@ struct Overflow {
 /// they are needed, so candidates are expanded in the exact (distance, id)
 /// order. Without an overflow region, or when it fills, the query goes to the
 /// general kernel.
-template <int ER, int EC, int PC, int ST>
+template <int ER, int EC, int PC, int ST, int RT = 1>
 __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s,
                                   const WaveLds& w, Visited& vis, uint32_t qi, uint64_t* runs) {
   const int lane = lane_id();
   const uint32_t ccap = 64u * EC;
   const uint32_t ef = a.ef;
 
-  for (uint32_t i = static_cast<uint32_t>(lane); i < (1u << vis.lds_log2); i += 64) w.hash[i] = kEmptySlot;
+  vis.clear_lds();
   vis.in_lds = 0;
   vis.in_hbm = 0;
   Work work;
@@ -1069,7 +1200,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
     vis.count(U);
     work.dist += U;
     ++work.exp0;
-    row_dists<PC, ST, 8, (ER >= 8 ? 2 : 4)>(ix, w, U);
+    row_dists<PC, ST, 8, (ER >= 8 ? 2 : 4), RT>(ix, w, U);
     wave_sync();
     const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
     const uint32_t id_u = w.nid[lane];
@@ -1124,12 +1255,15 @@ __device__ Handoff run_query(const DevIndexView& ix, const QueryArgs& a, const S
   if constexpr ((FM == kPerQuery || FM == kPerQueryDeleted) && ER > 1) {
     if (a.ef <= 32u * ER) return run_query<ER / 2, EC / 2, PC, FM, ST>(ix, a, s, w, vis, qi, runs);
   }
+  // Mixed launches keep one step per round trip: their kernels hold several
+  // capacity classes and are the tightest on registers.
+  constexpr int RT = (FM == kPerQuery || FM == kPerQueryDeleted) ? 1 : kRowTrip<ST>;
   if constexpr (FM == kPlain) {
-    return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
+    return plain_query<ER, PC, ST, false, RT>(ix, a, s, w, vis, qi);
   } else if constexpr (FM == kFiltered) {
-    return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi, runs);
+    return filtered_query<ER, EC, PC, ST, RT>(ix, a, s, w, vis, qi, runs);
   } else if constexpr (FM == kDeleted || FM == kPerQueryDeleted) {
-    return plain_query<ER, PC, ST, true>(ix, a, s, w, vis, qi);
+    return plain_query<ER, PC, ST, true, RT>(ix, a, s, w, vis, qi);
   } else {
     if (a.elig == nullptr) return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
     return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi, runs);
