@@ -98,10 +98,28 @@ def ground_truth(X: np.ndarray, Q: np.ndarray, k: int) -> tuple[np.ndarray, floa
     return out, agree / sample, info
 
 
-def measured_traffic(ef: int, args) -> dict:
+def tune_report(report: str, fields) -> dict:
+    """The autotuner's measurements per ef and dimension: for every value a
+    dimension took, the fastest step time (ms) of any shape with that value."""
+    out: dict = {}
+    for line in report.splitlines():
+        if not line.startswith("ef="):
+            continue
+        kv = dict(item.split("=") for item in line.split())
+        ef, ms = kv["ef"], float(kv["ms"])
+        for f in fields:
+            d = out.setdefault(ef, {}).setdefault(f, {})
+            d[kv[f]] = round(min(ms, d.get(kv[f], float("inf"))), 4)
+    return out
+
+
+def measured_traffic(ef: int, args, kernel_ms: float) -> dict:
     """HBM bytes per fast-kernel launch from the committed rocprofv3 counter
     runs (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration) on
-    this workload, when they cover this ef and shape; otherwise null."""
+    this workload, when they cover this ef and shape and still describe the
+    kernel that ran: the profile records the kernel's duration in its own run,
+    and a duration measured here more than 10 % away from it means the kernel
+    changed since (traffic null, with the reason)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "fast_search_mixture1M_traffic.json")
     try:
         with open(path) as f:
@@ -109,10 +127,15 @@ def measured_traffic(ef: int, args) -> dict:
         rec = t["kernels"][str(ef)]
         if args.n != 1_000_000 or args.nq != t["nq"] or args.k != 10 or args.set:
             raise KeyError("different workload")
-        return {"traffic": rec["hbm_bytes"], "traffic_source": "profiles/fast_search_mixture1M_ef64_ef128.md",
-                "traffic_over_algorithmic": round(rec["hbm_bytes"] / rec["algorithmic_bytes"], 3)}
     except (OSError, KeyError, ValueError):
         return {"traffic": None}
+    prof_ms = rec.get("kernel_ms")
+    if prof_ms is None or abs(kernel_ms - prof_ms) > 0.10 * prof_ms:
+        return {"traffic": None, "traffic_note": f"profile stale: kernel {kernel_ms:.3f} ms here, "
+                                                 f"{prof_ms} ms in {t.get('source', 'the profile')}"}
+    return {"traffic": rec["hbm_bytes"], "traffic_source": t.get("source", "profiles/"),
+            "traffic_over_algorithmic": round(rec["hbm_bytes"] / rec["algorithmic_bytes"], 3),
+            "traffic_profile_kernel_ms": prof_ms}
 
 
 def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3, max_cost: int = 0) -> dict:
@@ -232,12 +255,16 @@ def cpu_baseline(index_path: str, Q: np.ndarray, truth: np.ndarray, k: int, efs,
     when X is given, the reference's add_batch of X on the same cores (the
     build baseline; the time includes writing the file)."""
     sys.path.insert(0, str(ROOT / "oracle"))
-    from oracle import REF_SO, Reference
+    from oracle import REF_BENCH_SO, REF_SO, Reference
 
     cores = len(os.sched_getaffinity(0))
-    if not REF_SO.exists():
+    # The reference as its bench preset builds it (-O3 -march=native, pinned to the AVX-512 level
+    # x86-64-v4), else the portable checker build (-O3, per-file ISA flags).
+    so, flags = ((REF_BENCH_SO, "-O3 -march=x86-64-v4") if REF_BENCH_SO.exists()
+                 else (REF_SO, "-O3, -mavx2 -mfma / -mavx512f -mfma on the kernel files only"))
+    if not so.exists():
         return None
-    ref = Reference()
+    ref = Reference(so)
     ref_build_s = None
     if X is not None:
         path = os.path.join(tempfile.gettempdir(), "hnsw_bench_reference_build.idx")
@@ -265,7 +292,7 @@ def cpu_baseline(index_path: str, Q: np.ndarray, truth: np.ndarray, k: int, efs,
         points.append((ef, rec, nq / dt))
         log(f"cpu reference ef={ef} recall={rec:.4f} qps={nq / dt:,.0f} ({cores} threads)")
     return {"value": qps_at(points), "unit": "queries/s", "cores": cores, "kind": "reference",
-            "isa": ref.active_isa(), "reference_build_s": round(ref_build_s, 2) if ref_build_s else None,
+            "isa": ref.active_isa(), "build": f"{so.name}: {flags}", "reference_build_s": round(ref_build_s, 2) if ref_build_s else None,
             "sample": f"first {nq} of the {Q.shape[0]} queries, ef sweep {list(efs)}, search_batch on {cores} threads",
             "points": [{"ef": e, "recall": round(r, 5), "qps": round(q, 1)} for e, r, q in points],
             "_rows": rows_by_ef}
@@ -526,9 +553,12 @@ def main() -> int:
     else:
         # The autotuner picks the launch shape per ef class on batches of the size timed below.
         t0 = time.perf_counter()
-        dev.autotune(Q, k, [min(e, 512) for e in efs])
-        tuned = {str(c): {f: dev.get_option(f"tuned_{f}_{c}") for f in ("waves_per_cu", "hash_log2", "generic_gather")}
+        report = dev.autotune(Q, k, [min(e, 512) for e in efs])
+        fields = ("waves_per_cu", "hash_log2", "generic_gather", "row_trip", "min_class")
+        tuned = {str(c): {f: dev.get_option(f"tuned_{f}_{c}") for f in fields}
                  for c in (64, 128, 256, 512) if dev.get_option(f"tuned_waves_per_cu_{c}") >= 0}
+        tuned = {"chosen": tuned, "by_dimension": tune_report(report, fields),
+                 "seconds": round(time.perf_counter() - t0, 2)}
         log(f"autotune in {time.perf_counter() - t0:.1f}s: {tuned}")
     stream = hnswgpu.Stream()
     d_q = hnswgpu.DeviceBuffer.of(Q)
@@ -635,6 +665,17 @@ def main() -> int:
         lat.append((time.perf_counter() - t) * 1e3)
     lat = np.array(lat[20:])
     del q1
+    lat_default = None
+    if tuned is not None:
+        # The same with the default launch shapes (the tuner's shapes off).
+        dev.set_option("use_tuned", 0)
+        ld = []
+        for i in range(220):
+            t = time.perf_counter()
+            ix.search_rows(Q[i % args.nq:i % args.nq + 1], k, op["ef"])
+            ld.append((time.perf_counter() - t) * 1e3)
+        dev.set_option("use_tuned", 1)
+        lat_default = np.array(ld[20:])
 
     cpu, parity = None, None
     if not args.no_cpu_baseline:
@@ -681,7 +722,7 @@ def main() -> int:
                                f"{args.nq} queries, M=16, ef_construction=200, k={k}, ef sweep {efs}",
                    "n": args.n, "nq": args.nq, "k": k, "efs": efs, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": op["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(op["hbm_gbs"] / HBM_PEAK_GBS, 4), **measured_traffic(op["ef"], args),
+                     "frac": round(op["hbm_gbs"] / HBM_PEAK_GBS, 4), **measured_traffic(op["ef"], args, op["kernel_ms"]),
                      "kernel": "fast_search", "at_ef": op["ef"], "kernel_ms": op["kernel_ms"],
                      "bytes_per_query": op["bytes_per_query"]},
         "cpu_baseline": cpu,
@@ -694,6 +735,8 @@ def main() -> int:
         **precision,
         "latency_ms": {"single_query_p50": round(float(np.percentile(lat, 50)), 4),
                        "single_query_p99": round(float(np.percentile(lat, 99)), 4),
+                       "single_query_p50_default_shapes": (round(float(np.percentile(lat_default, 50)), 4)
+                                                           if lat_default is not None else None),
                        "at_ef": op["ef"]},
         "points": list(per_ef.values()),
         "overflow_queries": int(overflow),

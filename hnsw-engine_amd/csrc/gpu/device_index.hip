@@ -338,11 +338,18 @@ void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq) {
   r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(ef, h.view.count, h.wide_hash != 0);
   r.generic_gather = h.generic_gather;
   r.wide_hash = h.wide_hash;
+  // Reduced rows take half (bf16) or a quarter (fp8) of the registers of fp32
+  // ones, so two 16-row steps share a memory round trip there at no cost in
+  // occupancy; fp32 rows would drop a wavefront per SIMD (measured slower).
+  r.row_trip = h.row_trip != 0 ? h.row_trip : (h.view.storage != 0 ? 2 : 1);
+  r.min_class = h.min_class;
   const hnswgpu_index::Tuned& t = h.tuned[ef_class_index(ef)];
   if (h.use_tuned && t.valid) {
     waves_per_cu = t.waves_per_cu;
     r.hash_log2 = t.hash_log2;
     r.generic_gather = t.generic_gather;
+    r.row_trip = t.row_trip;
+    r.min_class = t.min_class;
   }
   const size_t per_wave = static_cast<size_t>(gpu::lds_words_per_wave(r.hash_log2, h.view.padded)) * 4u;
   r.waves_per_block = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / per_wave)));
@@ -695,36 +702,60 @@ int hnswgpu_autotune(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t k,
         if (ef > hnswamd::gpu::kFastMaxEf) continue;  // the general kernel has no launch shape to tune
         const uint32_t cls = hnswamd::dev::ef_class_index(static_cast<uint32_t>(ef));
         hnswgpu_index::Tuned best;
+        auto time_shape = [&](hnswgpu_index::Tuned cand) {
+          cand.valid = true;
+          h->tuned[cls] = cand;
+          h->use_tuned = 1;
+          search_device(*h, c, d_qs, nullptr, nq, k, ef, nullptr, 0, 0, d_out, nullptr, st);  // warm-up
+          check(hipEventRecord(e0, st), "hipEventRecord");
+          search_device(*h, c, d_qs, nullptr, nq, k, ef, nullptr, 0, 0, d_out, nullptr, st);
+          check(hipEventRecord(e1, st), "hipEventRecord");
+          check(hipEventSynchronize(e1), "hipEventSynchronize");
+          float ms = 0.0f;
+          check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+          cand.ms = ms;
+          if (!best.valid || cand.ms < best.ms) best = cand;
+          text += "ef=" + std::to_string(ef) + " waves_per_cu=" + std::to_string(cand.waves_per_cu) + " hash_log2=" +
+                  std::to_string(cand.hash_log2) + " generic_gather=" + std::to_string(cand.generic_gather) +
+                  " row_trip=" + std::to_string(cand.row_trip) + " min_class=" + std::to_string(cand.min_class) +
+                  " ms=" + std::to_string(ms) + "\n";
+        };
+        // Stage 1: occupancy (wavefronts per CU), LDS visited-hash size and row loop, at the
+        // storage's default rows per round trip and the ef's own capacity class.
+        const int default_trip = h->row_trip != 0 ? h->row_trip : (h->view.storage != 0 ? 2 : 1);
         for (uint32_t wpc : {8u, 12u, 16u, 20u, 24u}) {
           for (uint32_t hl : {10u, 11u, 12u, 13u}) {
             for (int gather : {0, 1}) {
               if (gather == 1 && h->view.padded != 128) continue;  // no fixed-width loop to compare with
               hnswgpu_index::Tuned cand;
-              cand.valid = true;
               cand.waves_per_cu = wpc;
               cand.hash_log2 = hl;
               cand.generic_gather = gather;
-              h->tuned[cls] = cand;
-              h->use_tuned = 1;
-              search_device(*h, c, d_qs, nullptr, nq, k, ef, nullptr, 0, 0, d_out, nullptr, st);  // warm-up
-              check(hipEventRecord(e0, st), "hipEventRecord");
-              search_device(*h, c, d_qs, nullptr, nq, k, ef, nullptr, 0, 0, d_out, nullptr, st);
-              check(hipEventRecord(e1, st), "hipEventRecord");
-              check(hipEventSynchronize(e1), "hipEventSynchronize");
-              float ms = 0.0f;
-              check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-              cand.ms = ms;
-              if (!best.valid || cand.ms < best.ms) best = cand;
-              text += "ef=" + std::to_string(ef) + " waves_per_cu=" + std::to_string(wpc) + " hash_log2=" +
-                      std::to_string(hl) + " generic_gather=" + std::to_string(gather) + " ms=" + std::to_string(ms) +
-                      "\n";
+              cand.row_trip = default_trip;
+              cand.min_class = cls;
+              time_shape(cand);
             }
           }
+        }
+        // Stage 2, around the best shape: rows per memory round trip (the gather width:
+        // 16 or 32 rows, fixed-width row loop only) and the capacity class (the ef's own,
+        // or the next larger one: more register rows for the result and candidate arrays).
+        const hnswgpu_index::Tuned base = best;
+        if (h->view.padded == 128 && base.generic_gather == 0) {
+          hnswgpu_index::Tuned cand = base;
+          cand.row_trip = base.row_trip == 1 ? 2 : 1;
+          time_shape(cand);
+        }
+        if (cls < 3) {
+          hnswgpu_index::Tuned cand = best;
+          cand.min_class = cls + 1;
+          time_shape(cand);
         }
         h->tuned[cls] = best;
         text += "best ef=" + std::to_string(ef) + " waves_per_cu=" + std::to_string(best.waves_per_cu) +
                 " hash_log2=" + std::to_string(best.hash_log2) + " generic_gather=" +
-                std::to_string(best.generic_gather) + " ms=" + std::to_string(best.ms) + "\n";
+                std::to_string(best.generic_gather) + " row_trip=" + std::to_string(best.row_trip) +
+                " min_class=" + std::to_string(best.min_class) + " ms=" + std::to_string(best.ms) + "\n";
       }
     } catch (...) {
       h->use_tuned = saved_use;
@@ -778,6 +809,12 @@ int hnswgpu_set_option(hnswgpu_index* h, const char* name, int64_t value) {
       h->waves_per_cu = static_cast<uint32_t>(value);
     } else if (n == "generic_gather") {
       h->generic_gather = value != 0 ? 1 : 0;
+    } else if (n == "row_trip") {
+      if (value < 0 || value > 2) throw std::invalid_argument("row_trip must be 0 (default), 1 or 2");
+      h->row_trip = static_cast<int>(value);
+    } else if (n == "min_class") {
+      if (value < 0 || value > 3) throw std::invalid_argument("min_class must be 0..3");
+      h->min_class = static_cast<uint32_t>(value);
     } else if (n == "use_tuned") {
       h->use_tuned = value != 0 ? 1 : 0;
     } else if (n == "wide_hash") {
@@ -863,6 +900,10 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
       *value = static_cast<int64_t>(static_cast<double>(ms) * 1e6);
     } else if (n == "generic_gather") {
       *value = h->generic_gather;
+    } else if (n == "row_trip") {
+      *value = h->row_trip;
+    } else if (n == "min_class") {
+      *value = h->min_class;
     } else if (n == "use_tuned") {
       *value = h->use_tuned;
     } else if (n == "wide_hash") {
@@ -881,6 +922,10 @@ int hnswgpu_get_option(hnswgpu_index* h, const char* name, int64_t* value) {
         *value = t.valid ? static_cast<int64_t>(t.hash_log2) : -1;
       } else if (field == "generic_gather") {
         *value = t.valid ? static_cast<int64_t>(t.generic_gather) : -1;
+      } else if (field == "row_trip") {
+        *value = t.valid ? static_cast<int64_t>(t.row_trip) : -1;
+      } else if (field == "min_class") {
+        *value = t.valid ? static_cast<int64_t>(t.min_class) : -1;
       } else if (field == "us") {
         *value = t.valid ? static_cast<int64_t>(t.ms * 1e3) : -1;
       } else {

@@ -174,6 +174,8 @@ struct hnswgpu_index {
     bool valid = false;
     uint32_t waves_per_cu = 16, hash_log2 = 11;
     int generic_gather = 0;
+    int row_trip = 1;         // 16-row steps per memory round trip
+    uint32_t min_class = 0;   // capacity class the ef class runs in (>= its own)
     double ms = 0.0;
   };
   Tuned tuned[4];
@@ -182,6 +184,8 @@ struct hnswgpu_index {
   // options
   int generic_gather = 0;
   int wide_hash = 0;
+  int row_trip = 0;           // 0: 2 for a reduced copy, 1 for fp32 rows (measured defaults)
+  uint32_t min_class = 0;
   uint32_t spill_waves = 64;
   uint32_t visit_log2 = 15;
   uint32_t waves_per_cu = 16;

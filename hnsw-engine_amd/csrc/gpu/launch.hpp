@@ -96,6 +96,8 @@ struct SearchRequest {
   int force_spill;           // route every query through the general kernel
   int generic_gather;        // 1: the row loop for any padded dim even where a fixed-width one exists
   int wide_hash;             // 1: 32-bit LDS visited entries even where 16-bit ones hold the ids
+  int row_trip;              // 16-row steps per memory round trip in the layer-0 expansion (1 or 2)
+  uint32_t min_class;        // smallest capacity class to run (0..3: ef <= 64 / 128 / 256 / 512)
 };
 
 /// Build the reduced copy of the rows for `storage` (1 = bf16, 2 = fp8) into

@@ -229,14 +229,6 @@ __device__ __forceinline__ void row_dists(const DevIndexView& ix, const WaveLds&
   }
 }
 
-#ifndef HNSWGPU_F32_ROW_TRIP
-#define HNSWGPU_F32_ROW_TRIP 1
-#endif
-/// 16-row steps per round trip in the layer-0 expansion: a reduced copy's rows
-/// take half (bf16) or a quarter (fp8) of the registers of fp32 ones, so two
-/// steps share one trip there at no cost in occupancy.
-template <int ST>
-constexpr int kRowTrip = ST == kStoreF32 ? HNSWGPU_F32_ROW_TRIP : 2;
 
 template <int ctrl, int row_mask>
 __device__ __forceinline__ uint64_t dpp_min64(uint64_t v) {
@@ -1249,15 +1241,12 @@ enum FilterMode : int { kPlain = 0, kFiltered = 1, kPerQuery = 2, kDeleted = 3, 
 /// One query. In a mixed launch (kPerQuery, kPerQueryDeleted) the kernel is
 /// built for the largest capacity class of the batch and a query with a
 /// smaller ef runs the code of its own class.
-template <int ER, int EC, int PC, int FM, int ST>
+template <int ER, int EC, int PC, int FM, int ST, int RT>
 __device__ Handoff run_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s, const WaveLds& w,
                              Visited& vis, uint32_t qi, uint64_t* runs) {
   if constexpr ((FM == kPerQuery || FM == kPerQueryDeleted) && ER > 1) {
-    if (a.ef <= 32u * ER) return run_query<ER / 2, EC / 2, PC, FM, ST>(ix, a, s, w, vis, qi, runs);
+    if (a.ef <= 32u * ER) return run_query<ER / 2, EC / 2, PC, FM, ST, RT>(ix, a, s, w, vis, qi, runs);
   }
-  // Mixed launches keep one step per round trip: their kernels hold several
-  // capacity classes and are the tightest on registers.
-  constexpr int RT = (FM == kPerQuery || FM == kPerQueryDeleted) ? 1 : kRowTrip<ST>;
   if constexpr (FM == kPlain) {
     return plain_query<ER, PC, ST, false, RT>(ix, a, s, w, vis, qi);
   } else if constexpr (FM == kFiltered) {
@@ -1270,7 +1259,7 @@ __device__ Handoff run_query(const DevIndexView& ix, const QueryArgs& a, const S
   }
 }
 
-template <int ER, int EC, int PC, int FM, int ST>
+template <int ER, int EC, int PC, int FM, int ST, int RT>
 __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView ix, SearchRequest r,
                                                                    SearchScratch s) {
   extern __shared__ __align__(16) uint32_t lds[];
@@ -1299,7 +1288,7 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView 
     if (!r.force_spill) {
       const QueryArgs a = query_args(r, qi);
       // A task whose ef exceeds the register capacity goes to the general kernel.
-      if (a.ef <= 64u * ER) h = run_query<ER, EC, PC, FM, ST>(ix, a, s, w, vis, qi, runs);
+      if (a.ef <= 64u * ER) h = run_query<ER, EC, PC, FM, ST, RT>(ix, a, s, w, vis, qi, runs);
     }
     if (h != Handoff::kNone && lane == 0) {
       const uint32_t slot = atomicAdd(&s.counters[1], 1u);
@@ -1310,27 +1299,44 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView 
 }
 
 
-template <int ER, int EC, int PC, int FM, int ST>
+template <int ER, int EC, int PC, int FM, int ST, int RT>
 hipError_t run_fast(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
   const size_t lds = static_cast<size_t>(r.waves_per_block) * lds_words_per_wave(r.hash_log2, ix.padded) * 4u;
   static bool configured = false;
   if (!configured) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fast_search<ER, EC, PC, FM, ST>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fast_search<ER, EC, PC, FM, ST, RT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
     configured = true;
   }
-  hipLaunchKernelGGL((fast_search<ER, EC, PC, FM, ST>), dim3(r.blocks), dim3(r.waves_per_block * 64), lds,
+  hipLaunchKernelGGL((fast_search<ER, EC, PC, FM, ST, RT>), dim3(r.blocks), dim3(r.waves_per_block * 64), lds,
                      st, ix, r, s);
   return hipGetLastError();
 }
 
+/// Capacity class: the smallest whose register arrays hold ef, or a larger
+/// one the request asks for (min_class, a tuner dimension: 0 = ef <= 64,
+/// 1 = <= 128, 2 = <= 256, 3 = <= 512).
+template <int PC, int FM, int ST, int RT>
+hipError_t pick_class(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
+  const uint32_t own = r.ef <= 64 ? 0u : r.ef <= 128 ? 1u : r.ef <= 256 ? 2u : 3u;
+  const uint32_t cls = own > r.min_class ? own : (r.min_class < 3u ? r.min_class : 3u);
+  if (cls == 0) return run_fast<1, 2, PC, FM, ST, RT>(ix, r, s, st);
+  if (cls == 1) return run_fast<2, 4, PC, FM, ST, RT>(ix, r, s, st);
+  if (cls == 2) return run_fast<4, 8, PC, FM, ST, RT>(ix, r, s, st);
+  return run_fast<8, 16, PC, FM, ST, RT>(ix, r, s, st);
+}
+
+/// Rows per memory round trip in the layer-0 expansion (16 * row_trip; a
+/// tuner dimension): two 16-row steps share a trip only where the row width
+/// is fixed, and not in mixed launches, whose kernels hold several capacity
+/// classes and are the tightest on registers.
 template <int PC, int FM, int ST>
 hipError_t pick_capacity(const DevIndexView& ix, const SearchRequest& r, SearchScratch& s, hipStream_t st) {
-  if (r.ef <= 64) return run_fast<1, 2, PC, FM, ST>(ix, r, s, st);
-  if (r.ef <= 128) return run_fast<2, 4, PC, FM, ST>(ix, r, s, st);
-  if (r.ef <= 256) return run_fast<4, 8, PC, FM, ST>(ix, r, s, st);
-  return run_fast<8, 16, PC, FM, ST>(ix, r, s, st);
+  if constexpr (PC > 0 && FM != kPerQuery && FM != kPerQueryDeleted) {
+    if (r.row_trip == 2) return pick_class<PC, FM, ST, 2>(ix, r, s, st);
+  }
+  return pick_class<PC, FM, ST, 1>(ix, r, s, st);
 }
 
 template <int PC, int ST>

@@ -100,7 +100,7 @@ std::vector<uint64_t> shape_words(const hnswamd::gpu::SearchRequest& r, const hn
   return {P(r.tasks), P(r.nq_dev), P(r.queries), P(r.qscale), r.dim, r.nq, r.k, r.ef, P(r.elig),
           static_cast<uint64_t>(r.deleted_only), P(r.out), P(r.counts), P(r.stats), r.hash_log2, r.waves_per_block,
           r.blocks, static_cast<uint64_t>(r.force_spill), static_cast<uint64_t>(r.generic_gather),
-          static_cast<uint64_t>(r.wide_hash),
+          static_cast<uint64_t>(r.wide_hash), static_cast<uint64_t>(r.row_trip), r.min_class,
           P(s.counters), P(s.visit_tables), s.visit_log2, P(s.overflow), P(s.qprep), P(s.spill_words),
           P(s.spill_heaps), s.spill_heap_stride, s.spill_waves, s.spill_bitmap_words, P(s.cand_runs),
           P(ix.vecs), P(ix.l0), P(ix.upper), P(ix.upper_off), P(ix.labels), ix.count, ix.padded, ix.M, ix.M0,

@@ -260,12 +260,15 @@ int hnswgpu_server_free(hnswgpu_server* s);
 
 /* Autotuner (no reference counterpart): for each ef in efs (one per capacity
  * class 64 / 128 / 256 / 512 suffices), times the search of the nq sample
- * queries (host, k results) with HIP events for every launch shape --
- * wavefronts per CU {8, 12, 16, 20, 24} x LDS visited-hash words 2^{10..13} x
- * row gather (fixed-width loop for padded dim 128, or the generic one) -- and
- * keeps the fastest per class; searches then use it ("use_tuned" = 1). One
- * wavefront serves one query in every shape. Results do not depend on the
- * shape. The best shape depends on the batch size (a batch smaller than the
+ * queries (host, k results) with HIP events for every launch shape in two
+ * stages: wavefronts per CU {8, 12, 16, 20, 24} x LDS visited-hash words
+ * 2^{10..13} x row loop (fixed-width for padded dim 128, or the generic one);
+ * then, around the best of those, rows per memory round trip (16 or 32: the
+ * gather width of the row loads, fixed-width loop only) and capacity class
+ * (the ef's own, or the next larger: more register rows for the result and
+ * candidate arrays) -- and keeps the fastest per class; searches then use it
+ * ("use_tuned" = 1). One wavefront serves one query in every shape. Results
+ * do not depend on the shape. The best shape depends on the batch size (a batch smaller than the
  * resident wavefronts is not bound by occupancy), so the sample should be as
  * large as the batches to be served. report (nullable) receives one line per
  * measurement. */
@@ -277,6 +280,9 @@ int hnswgpu_autotune(hnswgpu_index* h, const float* queries, uint64_t nq, uint64
  *   "waves_per_cu"   resident wavefronts per CU of the persistent grid
  *   "visit_log2"     slots of each wavefront's HBM visited table
  *   "generic_gather" 1 = the generic row loop even for padded dim 128
+ *   "row_trip"       16-row steps per memory round trip, fixed-width loop only
+ *                    (0 = 2 for a bf16 / fp8 traversal copy, 1 for fp32 rows)
+ *   "min_class"      smallest capacity class to run (0..3: ef <= 64/128/256/512)
  *   "wide_hash"      1 = 32-bit LDS visited entries (default: 16-bit when ids fit)
  *   "use_tuned"      1 = the autotuner's launch shapes (set by hnswgpu_autotune)
  *   "storage"        traversal copy, below

@@ -3,7 +3,9 @@
 * ``Oracle``     -- oracle/liboracle.so, the project's CPU restatement of the
                     reference's query path (oracle/oracle.cpp);
 * ``Reference``  -- oracle/_ref/libhnswref.so, the reference itself compiled
-                    from its sources plus oracle/ref_shim.cpp.
+                    from its sources plus oracle/ref_shim.cpp (portable flags:
+                    the checker), or oracle/_ref/libhnswref_bench.so (the same
+                    sources at -O3 -march=x86-64-v4: the CPU baseline).
 
 Only tests/, __graft_entry__.smoke() and the CPU-baseline leg of bench.py use
 this module.
@@ -20,6 +22,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "liboracle.so"
 REF_SO = HERE / "_ref" / "libhnswref.so"
+REF_BENCH_SO = HERE / "_ref" / "libhnswref_bench.so"
 ROW = np.dtype([("label", "<u8"), ("distance", "<f4"), ("reserved", "<u4")])
 ORDER = {"scalar": 0, "avx2": 1, "avx512": 2}
 p, u64, i64 = C.c_void_p, C.c_uint64, C.c_int64
@@ -122,10 +125,10 @@ OracleIndex.label_dists = _label_dists
 class Reference:
     """The reference library compiled from its own sources (oracle/_ref)."""
 
-    def __init__(self):
-        if not REF_SO.exists():
-            raise FileNotFoundError(f"{REF_SO} missing: make -C oracle ref (needs the reference sources)")
-        lib = C.CDLL(str(REF_SO))
+    def __init__(self, so=REF_SO):
+        if not Path(so).exists():
+            raise FileNotFoundError(f"{so} missing: make -C oracle ref (needs the reference sources)")
+        lib = C.CDLL(str(so))
         lib.ref_last_error.restype = C.c_char_p
         lib.ref_set_isa.argtypes = [C.c_char_p]
         lib.ref_active_isa.argtypes = [p, C.c_int]
