@@ -216,9 +216,9 @@ __device__ __forceinline__ void row_dists_metric(const DevIndexView& ix, const W
   }
 }
 
-/// G / BG: group sizes of the run-time-width fp32 / bf16 row loops. The
-/// largest capacity class (ER = 8) takes smaller groups: its registers would
-/// otherwise spill to scratch.
+/// G / BG: group sizes of the run-time-width fp32 / bf16 row loops (chunks /
+/// 16-byte pieces per lane loaded before their arithmetic; a bf16 row of up to
+/// 128 elements is one group of 4 pieces, so one memory round trip).
 /// RT: 16-row steps per memory round trip where the row width is fixed.
 template <int PC, int ST = kStoreF32, int G = 8, int BG = 4, int RT = 1>
 __device__ __forceinline__ void row_dists(const DevIndexView& ix, const WaveLds& w, uint32_t U) {
@@ -533,7 +533,8 @@ __device__ uint64_t start_query(const DevIndexView& ix, const SearchScratch& s, 
 /// Each batch of 64 keys is scored at once and only keys below the current
 /// k-th are inserted.
 template <int XE, int ER, int PC, int G>
-__device__ void rerank_top(const DevIndexView& ix, const WaveLds& w, WaveSorted<ER>& R, uint32_t sR, uint32_t k) {
+__device__ __forceinline__ WaveSorted<ER> rerank_top(const DevIndexView& ix, const WaveLds& w, WaveSorted<ER> R,
+                                                     uint32_t sR, uint32_t k) {
   const int lane = lane_id();
   const uint32_t cap = k < sR ? k : sR;
   WaveSorted<XE> X;
@@ -560,11 +561,17 @@ __device__ void rerank_top(const DevIndexView& ix, const WaveLds& w, WaveSorted<
   }
 #pragma unroll
   for (int e = 0; e < ER; ++e) R.v[e] = e < XE ? X.v[e < XE ? e : 0] : kKeyInf;
+  return R;
 }
 
+/// Everything is taken (and R returned) by value: a reference would keep the
+/// caller's R, index view and LDS view in scratch memory for the whole search
+/// wherever this is not inlined (the largest capacity class), not just around
+/// the call.
 template <int ER, int PC, int G>
-__device__ void rerank_exact(const DevIndexView& ix, const WaveLds& w, WaveSorted<ER>& R, uint32_t sR, uint32_t k) {
-  rerank_top<ER, ER, PC, G>(ix, w, R, sR, k);
+__device__ WaveSorted<ER> rerank_exact(const DevIndexView ix, const WaveLds w, WaveSorted<ER> R, uint32_t sR,
+                                       uint32_t k) {
+  return rerank_top<ER, ER, PC, G>(ix, w, R, sR, k);
 }
 
 /// A node's layer-0 list in registers: lane j holds entry j + 1, `cnt` the
@@ -849,6 +856,9 @@ __device__ bool admit_merged(WaveSorted<ER>& R, uint32_t (&fl)[ER], uint32_t ef,
 /// result of a full R is dead (never expanded); D's head being dead ends the
 /// search, as Alg. 2's stop rule does. A live key pushed out of a full D hands
 /// the query to the general kernel.
+#ifndef HNSWGPU_MERGE_MAX_ER
+#define HNSWGPU_MERGE_MAX_ER 8
+#endif
 template <int ER, int PC, int ST, bool DEL = false, int RT = 1>
 __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s,
                                const WaveLds& w, Visited& vis, uint32_t qi) {
@@ -859,7 +869,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
   vis.in_lds = 0;
   vis.in_hbm = 0;
   Work work;
-  const uint64_t ep = start_query<PC, ST, (ER >= 8 ? 2 : 4)>(ix, s, w, qi, work);
+  const uint64_t ep = start_query<PC, ST, 4>(ix, s, w, qi, work);
 
   WaveSorted<ER> R;
   R.clear();
@@ -933,7 +943,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
     vis.count(U);
     work.dist += U;
     ++work.exp0;
-    row_dists<PC, ST, 8, (ER >= 8 ? 2 : 4), RT>(ix, w, U);
+    row_dists<PC, ST, 8, 4, RT>(ix, w, U);
     wave_sync();
     const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
     const uint32_t id_u = w.nid[lane];
@@ -943,7 +953,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
     // worst distance need the sequential order. (At ef <= 64 the merge's
     // registers would cost a wavefront per SIMD: the sequential loop is kept.)
     // With deletions only when every candidate is eligible.
-    if constexpr (ER >= 2) {
+    if constexpr (ER >= 2 && ER <= HNSWGPU_MERGE_MAX_ER) {
       if (sR == ef && (cand & (cand - 1)) != 0 && (!DEL || (cand & ~ballot(el_u != 0u)) == 0) &&
           admit_merged<ER>(R, fl, ef, tau, cand, key_of(od_u, id_u))) {
         n_ties = 0;
@@ -995,7 +1005,7 @@ __device__ Handoff plain_query(const DevIndexView& ix, const QueryArgs& a, const
   if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
   vis.finish();
   if (result != Handoff::kNone) return result;
-  if constexpr (ST != kStoreF32) rerank_exact<ER, PC, (ER >= 8 && ST == kStoreBF16 ? 4 : 8)>(ix, w, R, sR, a.k);
+  if constexpr (ST != kStoreF32) R = rerank_exact<ER, PC, (ER >= 4 && ST == kStoreBF16 ? 4 : 8)>(ix, w, R, sR, a.k);
   write_results<ER>(ix, a, R, sR, work);
   return Handoff::kNone;
 }
@@ -1136,7 +1146,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
   vis.in_lds = 0;
   vis.in_hbm = 0;
   Work work;
-  const uint64_t ep = start_query<PC, ST, (ER >= 8 ? 2 : 4)>(ix, s, w, qi, work);
+  const uint64_t ep = start_query<PC, ST, 4>(ix, s, w, qi, work);
 
   WaveSorted<ER> R;
   WaveSorted<EC> C;
@@ -1192,7 +1202,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
     vis.count(U);
     work.dist += U;
     ++work.exp0;
-    row_dists<PC, ST, 8, (ER >= 8 ? 2 : 4), RT>(ix, w, U);
+    row_dists<PC, ST, 8, 4, RT>(ix, w, U);
     wave_sync();
     const uint32_t od_u = static_cast<uint32_t>(lane) < U ? w.nd[lane] : kOrdInf;
     const uint32_t id_u = w.nid[lane];
@@ -1228,7 +1238,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
   if (vis.spilled && lane == 0) atomicAdd(&s.counters[3], 1u);
   vis.finish();
   if (result != Handoff::kNone) return result;
-  if constexpr (ST != kStoreF32) rerank_exact<ER, PC, (ER >= 8 && ST == kStoreBF16 ? 4 : 8)>(ix, w, R, sR, a.k);
+  if constexpr (ST != kStoreF32) R = rerank_exact<ER, PC, (ER >= 4 && ST == kStoreBF16 ? 4 : 8)>(ix, w, R, sR, a.k);
   write_results<ER>(ix, a, R, sR, work);
   return Handoff::kNone;
 }
