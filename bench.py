@@ -138,7 +138,7 @@ def measured_traffic(ef: int, args, kernel_ms: float) -> dict:
             "traffic_profile_kernel_ms": prof_ms}
 
 
-def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3, max_cost: int = 0) -> dict:
+def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3, max_cost: int = 0, heavy_cost: int = 0) -> dict:
     """The serving loop under the mixed load of a production index: requests
     of log-uniform batch size 1..512, k in {1, 10, 100}, ef in {32..512},
     30 % with a 10 %-density BitsetFilter, half traversing the BF16 copy, 1 %
@@ -174,7 +174,7 @@ def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3, max_cost: i
                 "launches": int(st["launches"]), "mean_queries_per_launch": round(st["mean_batch"], 1),
                 "graph_launches": int(st["graph_launches"])}
 
-    srv = hnswgpu.Server(dev, slots=slots, max_batch=4096, storages=(1,), max_cost=max_cost)
+    srv = hnswgpu.Server(dev, slots=slots, max_batch=4096, storages=(1,), max_cost=max_cost, heavy_cost=heavy_cost)
     # Warm-up: every slot captures one graph per (storage, ef class, kernel)
     # on first use, the kernel being the mixed one when a launch holds a
     # filtered request and the deletions side-list one when it holds none.
@@ -198,14 +198,30 @@ def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3, max_cost: i
     closed = summary(srv.stats(), time.perf_counter() - t0, reqs)
     log(f"serving closed loop {closed}")
 
+    # A request is heavy when its queries cost at least the server's heavy_cost
+    # (max(ef, k) / filter density >= 1,024: the filtered ones at ef >= 128);
+    # heavy requests form launches of their own.
+    def heavy(r):
+        return (max(r[2], r[1]) / (0.10 if r[3] is not None else 1.0)) >= 1024
+
+    def by_class(reqs, lat):
+        out = {}
+        for name, sel in (("cheap", lambda r: not heavy(r)), ("heavy", heavy)):
+            ls = np.array([l for r, l in zip(reqs, lat) if sel(r)])
+            if ls.size:
+                out[name] = {"requests": int(ls.size), "p50_ms": round(float(np.percentile(ls, 50)), 3),
+                             "p99_ms": round(float(np.percentile(ls, 99)), 3)}
+        return out
+
     def open_loop(rate, seconds_):
         reqs = make(max(100, int(rate * seconds_)))
         srv.reset_stats()
         pending: "queue.Queue[int | None]" = queue.Queue()
+        lat = []
 
         def waiter():
             while (t := pending.get()) is not None:
-                srv.wait(t)
+                lat.append(srv.wait_latency(t))
 
         th = threading.Thread(target=waiter)
         th.start()
@@ -221,6 +237,7 @@ def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3, max_cost: i
         th.join()
         out = summary(srv.stats(), time.perf_counter() - t0, reqs)
         out["offered_requests_per_s"] = round(rate, 1)
+        out["by_class"] = by_class(reqs, lat)
         return out
 
     saturated = closed["requests"] / closed["wall_s"]
@@ -463,6 +480,8 @@ def main() -> int:
     ap.add_argument("--serving-seconds", type=float, default=3.0)
     ap.add_argument("--serving-slots", type=int, default=SERVING_SLOTS)
     ap.add_argument("--serving-max-cost", type=int, default=0, help="cost units per launch (0: the server's default)")
+    ap.add_argument("--serving-heavy-cost", type=int, default=0,
+                    help="query cost from which requests form launches of their own (0: the server's default)")
     ap.add_argument("--serving-only", action="store_true", help="only the serving section, one line")
     ap.add_argument("--profile-brute", action="store_true",
                     help="profiling run of the brute force only (n x 128 data, nq queries, k), one line")
@@ -530,7 +549,7 @@ def main() -> int:
 
     if args.serving_only:
         print(json.dumps({"serving": serving_bench(ix, Q, args.serving_seconds, args.serving_slots,
-                                                   args.serving_max_cost)}), flush=True)
+                                                   args.serving_max_cost, args.serving_heavy_cost)}), flush=True)
         return 0
 
     if args.deleted > 0:
@@ -695,7 +714,7 @@ def main() -> int:
 
     # Last: it soft-deletes 1 % of the base.
     serving = None if args.no_serving else serving_bench(ix, Q, args.serving_seconds, args.serving_slots,
-                                                         args.serving_max_cost)
+                                                         args.serving_max_cost, args.serving_heavy_cost)
     del ix, dev
     precision = {}
     if not args.no_precision:

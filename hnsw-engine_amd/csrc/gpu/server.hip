@@ -64,6 +64,7 @@ struct Request {
   std::string error;
   bool finished = false;
   Clock::time_point t_submit;
+  double latency_ms = 0.0;  // submit to completion, set when finished
 };
 
 /// A run of one request's queries inside a launch.
@@ -507,7 +508,8 @@ void complete_loop(hnswgpu_server* srv, Slot* slot) {
       q.done += p.n;
       if (q.done == q.r.nq) {
         q.finished = true;
-        srv->latencies_ms.push_back(std::chrono::duration<double, std::milli>(now - q.t_submit).count());
+        q.latency_ms = std::chrono::duration<double, std::milli>(now - q.t_submit).count();
+        srv->latencies_ms.push_back(q.latency_ms);
         srv->queries_done += q.r.nq;
         srv->t_last = now;
       }
@@ -679,7 +681,7 @@ int hnswgpu_server_submit(hnswgpu_server* s, const hnswgpu_request* req, uint64_
   });
 }
 
-int hnswgpu_server_wait(hnswgpu_server* s, uint64_t ticket) {
+int hnswgpu_server_wait_latency(hnswgpu_server* s, uint64_t ticket, double* latency_ms) {
   std::shared_ptr<Request> q;
   const int rc = guarded([&] {
     if (s == nullptr) throw std::invalid_argument("server_wait: null server");
@@ -691,9 +693,12 @@ int hnswgpu_server_wait(hnswgpu_server* s, uint64_t ticket) {
     s->requests.erase(it);
   });
   if (rc != HNSWGPU_OK) return rc;
+  if (latency_ms != nullptr) *latency_ms = q->latency_ms;
   if (q->status != HNSWGPU_OK) hnswamd::set_last_error(q->error);
   return q->status;
 }
+
+int hnswgpu_server_wait(hnswgpu_server* s, uint64_t ticket) { return hnswgpu_server_wait_latency(s, ticket, nullptr); }
 
 int hnswgpu_server_stats(hnswgpu_server* s, double* out, uint64_t n) {
   return guarded([&] {

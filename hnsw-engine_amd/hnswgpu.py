@@ -97,6 +97,7 @@ def _load():
         "hnswgpu_server_create": ([p, p, p], C.c_int),
         "hnswgpu_server_submit": ([p, p, p], C.c_int),
         "hnswgpu_server_wait": ([p, u64], C.c_int),
+        "hnswgpu_server_wait_latency": ([p, u64, p], C.c_int),
         "hnswgpu_server_stats": ([p, p, u64], C.c_int),
         "hnswgpu_server_reset_stats": ([p], C.c_int),
         "hnswgpu_server_free": ([p], C.c_int),
@@ -330,6 +331,15 @@ class Server:
         q, w, out, counts = self._live.pop(ticket)
         _check(rc)
         return out.reshape(q.shape[0], -1), counts
+
+    def wait_latency(self, ticket: int) -> float:
+        """wait() returning the request's latency (ms, submit to completion)
+        instead of its rows (which are in the buffers submit made)."""
+        ms = C.c_double()
+        rc = _lib.hnswgpu_server_wait_latency(self._h, ticket, C.byref(ms))
+        self._live.pop(ticket)
+        _check(rc)
+        return ms.value
 
     def stats(self) -> dict:
         a = np.zeros(len(SERVER_STATS), np.float64)

@@ -246,6 +246,9 @@ int hnswgpu_server_submit(hnswgpu_server* s, const hnswgpu_request* request, uin
 /* Block until the request has completed; returns its status. Each ticket is
  * waited for once. Results equal hnswgpu_search_batch on the same arguments. */
 int hnswgpu_server_wait(hnswgpu_server* s, uint64_t ticket);
+/* hnswgpu_server_wait that also returns the request's latency (ms, submit to
+ * completion, as the statistics count it). */
+int hnswgpu_server_wait_latency(hnswgpu_server* s, uint64_t ticket, double* latency_ms);
 /* Statistics since the last reset, as doubles: [0] requests completed,
  * [1] queries completed, [2] launches, [3] mean queries per launch,
  * [4] p50 and [5] p99 request latency (ms, submit to completion), [6] max
