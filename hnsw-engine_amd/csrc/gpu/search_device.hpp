@@ -1129,7 +1129,7 @@ struct Overflow {
   }
 };
 
-/// Layer-0 search with an eligibility bitmap (filter and/or deletions): C and R/// Layer-0 search with an eligibility bitmap (filter and/or deletions): C and R
+/// Layer-0 search with an eligibility bitmap (filter and/or deletions): C and R
 /// are separate sorted arrays; C keeps its 64 * EC smallest entries in
 /// registers and the rest in the overflow, from which entries come back before
 /// they are needed, so candidates are expanded in the exact (distance, id)

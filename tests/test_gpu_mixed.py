@@ -82,6 +82,47 @@ def test_mixed_with_deletions_and_bf16(lib, mixed_index):
     dev.set_option("storage", 0)
 
 
+def test_mixed_deletions_without_filters_equal_oracle(lib, oracle, tmp_path):
+    """Unfiltered mixed launches on an index with soft deletions take the
+    per-query deletions kernel, whose ef classes nest (64 .. 512): with
+    deleted_path 2 (side list of admitted deleted nodes at any count) and 0
+    (the filtered kernel) every row equals the oracle's in the AVX-512 order
+    (src/search_layer.cpp:58-134, deleted nodes are traversed but never
+    admitted to the result)."""
+    import workloads
+
+    X = workloads.mixture_128(10_000, seed=24)
+    Q = workloads.mixture_128(400, seed=1024)
+    ix = lib.Index(128, "l2")
+    ix.add(X)
+    ix.save(tmp_path / "mixdel.idx")
+    orc = oracle.load(tmp_path / "mixdel.idx")
+    for label in range(5, 10_000, 89):
+        ix.mark_deleted(label)
+        orc.mark_deleted(label)
+    dev = ix.device()
+    rng = np.random.default_rng(4)
+    n = Q.shape[0]
+    ks = rng.choice([1, 10, 100], n)
+    efs = rng.choice([10, 40, 100, 150, 250, 300, 500], n)
+    fids = np.full(n, -1)
+    want = {}
+    for k, ef in sorted(set(zip(ks.tolist(), efs.tolist()))):
+        idx = np.nonzero((ks == k) & (efs == ef))[0]
+        want[(k, ef)] = (idx, orc.search(Q[idx], int(k), int(ef), order="avx512"))
+    assert len(want) > 15
+    for path in (2, 0):
+        dev.set_option("deleted_path", path)
+        got = dev.search_mixed(Q, ks, efs, fids, [])
+        for (k, ef), (idx, w) in want.items():
+            for j, i in enumerate(idx):
+                rows, c = got[i]
+                assert rows_equal(rows[None], w[j:j + 1]), (path, k, ef, i)
+                assert c == int((w[j]["label"] != lib.INVALID_LABEL).sum()), (path, k, ef, i)
+                assert not np.isin(rows["label"][:c].astype(np.int64), np.arange(5, 10_000, 89)).any()
+    dev.set_option("deleted_path", 1)
+
+
 def test_mixed_rejects_bad_tasks(lib, mixed_index):
     ix, X, Q = mixed_index
     dev = ix.device()
