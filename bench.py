@@ -267,10 +267,12 @@ def qps_at(points, target=0.95):
     return workloads.qps_at_recall(points, target)
 
 
-def cpu_baseline(index_path: str, Q: np.ndarray, truth: np.ndarray, k: int, efs, budget_s: float, X=None):
-    """The reference's search_batch on this host's cores (oracle/_ref), and,
-    when X is given, the reference's add_batch of X on the same cores (the
-    build baseline; the time includes writing the file)."""
+def cpu_baseline(index_path: str, Q: np.ndarray, truth: np.ndarray, k: int, efs, budget_s: float, X=None,
+                 lat_ef: int = 128):
+    """The reference's search_batch on this host's cores (oracle/_ref), its
+    single-query latency on one thread at lat_ef, and, when X is given, the
+    reference's add_batch of X on the same cores (the build baseline; the time
+    includes writing the file)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     from oracle import REF_BENCH_SO, REF_SO, Reference
 
@@ -308,10 +310,21 @@ def cpu_baseline(index_path: str, Q: np.ndarray, truth: np.ndarray, k: int, efs,
         rec = recall_at(out["label"].astype(np.int64), truth[:nq], k)
         points.append((ef, rec, nq / dt))
         log(f"cpu reference ef={ef} recall={rec:.4f} qps={nq / dt:,.0f} ({cores} threads)")
+    # One query per call on one thread, as the GPU's single-query latency is measured.
+    lat = []
+    for i in range(220):
+        t = time.perf_counter()
+        idx.search(Q[i:i + 1], k, lat_ef, threads=1)
+        lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat[20:])
+    log(f"cpu reference single query ef={lat_ef}: p50 {np.percentile(lat, 50):.4f} ms, p99 {np.percentile(lat, 99):.4f} ms")
     return {"value": qps_at(points), "unit": "queries/s", "cores": cores, "kind": "reference",
             "isa": ref.active_isa(), "build": f"{so.name}: {flags}", "reference_build_s": round(ref_build_s, 2) if ref_build_s else None,
             "sample": f"first {nq} of the {Q.shape[0]} queries, ef sweep {list(efs)}, search_batch on {cores} threads",
             "points": [{"ef": e, "recall": round(r, 5), "qps": round(q, 1)} for e, r, q in points],
+            "single_query_1thread_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                                        "p99": round(float(np.percentile(lat, 99)), 4), "at_ef": lat_ef,
+                                        "queries": int(lat.size)},
             "_rows": rows_by_ef}
 
 
@@ -698,7 +711,8 @@ def main() -> int:
 
     cpu, parity = None, None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(index_path, Q, truth, k, efs, args.cpu_budget_s, X=X if build is not None else None)
+        cpu = cpu_baseline(index_path, Q, truth, k, efs, args.cpu_budget_s, X=X if build is not None else None,
+                           lat_ef=op["ef"])
     if cpu is not None:
         # Row parity with the reference's own search_batch on the same file and queries: a row
         # mismatches if any label or distance bit differs (the GPU computes in the AVX-512 order).

@@ -609,13 +609,17 @@ __device__ __forceinline__ uint32_t fresh_from_list(const DevIndexView& ix, cons
       if (has && static_cast<uint32_t>(lane) > i && nb == vi) has = false;
     }
   }
+  // Eligibility words are loaded for the whole list before the visited probe,
+  // so their latency runs under the probe instead of after it.
+  uint32_t ew = 0;
+  if (elig != nullptr && has) ew = reinterpret_cast<const global_u32*>(reinterpret_cast<uintptr_t>(elig))[nb >> 5];
   const bool fresh = has && visit(nb);
   const uint64_t fm = ballot(fresh);
   wave_sync();
   if (fresh) {
     const uint32_t p = lanes_below(fm);
     w.nid[p] = nb;
-    if (elig != nullptr) w.nel[p] = elig_bit(elig, nb) ? 1u : 0u;
+    if (elig != nullptr) w.nel[p] = (ew >> (nb & 31)) & 1u;
   }
   wave_sync();
   return static_cast<uint32_t>(__popcll(fm));
