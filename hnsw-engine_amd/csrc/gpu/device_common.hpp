@@ -182,6 +182,43 @@ struct WaveSorted {
     }
     return out;
   }
+  /// Ascending sort of all 64 * E entries (entry 64e + l in row e, lane l) by
+  /// a bitonic network: partners more than 63 entries apart are in other rows
+  /// of the same lane (a register compare-exchange), nearer ones in other
+  /// lanes of the same row (a lane gather). E must be a power of two.
+  __device__ __forceinline__ void sort_all() {
+    static_assert((E & (E - 1)) == 0, "E must be a power of two");
+    const uint32_t lane = static_cast<uint32_t>(lane_id());
+#pragma unroll
+    for (uint32_t k = 2; k <= 64u * E; k <<= 1) {
+#pragma unroll
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        if (j >= 64) {
+          const uint32_t jr = j >> 6;
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            if ((static_cast<uint32_t>(e) & jr) == 0) {
+              const int f = e | static_cast<int>(jr);
+              // Direction: bit k of the lower index 64e + lane; k >= 128 here, so the row decides.
+              const bool up = ((static_cast<uint32_t>(e) * 64u) & k) == 0;
+              const uint64_t a = v[e], b = v[f < E ? f : e];
+              const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
+              v[e] = up ? lo : hi;
+              v[f < E ? f : e] = up ? hi : lo;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const uint64_t o = from_lane64(v[e], static_cast<int>(lane ^ j));
+            const uint32_t i = static_cast<uint32_t>(e) * 64u + lane;
+            const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
+            v[e] = keep_min ? (o < v[e] ? o : v[e]) : (o > v[e] ? o : v[e]);
+          }
+        }
+      }
+    }
+  }
   /// Remove element 0.
   __device__ __forceinline__ void pop_front() {
 #pragma unroll

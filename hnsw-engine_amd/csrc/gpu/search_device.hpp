@@ -564,6 +564,32 @@ __device__ __forceinline__ WaveSorted<ER> rerank_top(const DevIndexView& ix, con
   return R;
 }
 
+/// The same result in place, for the largest capacity class, where a second
+/// array of register rows would cost occupancy: each key's distance is
+/// replaced by its exact one, 64 rows at a time, and R is sorted, so its first
+/// min(k, sR) entries are the exactly smallest keys in (distance, id) order.
+/// Never inlined, so the sort's registers are not added to the search loop's.
+template <int ER, int PC, int G>
+__device__ __attribute__((noinline)) WaveSorted<ER> rerank_sorted(const DevIndexView ix, const WaveLds w,
+                                                                  WaveSorted<ER> R, uint32_t sR) {
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+#pragma unroll
+  for (int e = 0; e < ER; ++e) {
+    const uint32_t base = static_cast<uint32_t>(e) * 64u;
+    if (base < sR) {
+      const uint32_t cnt = sR - base < 64u ? sR - base : 64u;
+      wave_sync();
+      if (lane < cnt) w.nid[lane] = key_id(R.v[e]);
+      wave_sync();
+      row_dists<PC, kStoreF32, G>(ix, w, cnt);
+      wave_sync();
+      R.v[e] = lane < cnt ? key_of(w.nd[lane], w.nid[lane]) : kKeyInf;
+    }
+  }
+  R.sort_all();
+  return R;
+}
+
 /// Everything is taken (and R returned) by value: a reference would keep the
 /// caller's R, index view and LDS view in scratch memory for the whole search
 /// wherever this is not inlined (the largest capacity class), not just around
@@ -571,7 +597,11 @@ __device__ __forceinline__ WaveSorted<ER> rerank_top(const DevIndexView& ix, con
 template <int ER, int PC, int G>
 __device__ WaveSorted<ER> rerank_exact(const DevIndexView ix, const WaveLds w, WaveSorted<ER> R, uint32_t sR,
                                        uint32_t k) {
-  return rerank_top<ER, ER, PC, G>(ix, w, R, sR, k);
+  if constexpr (ER >= 8) {
+    return rerank_sorted<ER, PC, G>(ix, w, R, sR);
+  } else {
+    return rerank_top<ER, ER, PC, G>(ix, w, R, sR, k);
+  }
 }
 
 /// A node's layer-0 list in registers: lane j holds entry j + 1, `cnt` the
