@@ -444,6 +444,11 @@ def reduced_precision_run(args, which: str) -> dict:
         dev.set_option("storage", 0)
         a, b = per["fp32"]["qps_at_recall_0.95"], per["reduced"]["qps_at_recall_0.95"]
         per["reduced_over_fp32"] = round(b / a, 3) if a and b else None
+        # The same ef on both copies, with each side's recall: also defined where neither
+        # reaches the recall target.
+        per["by_ef"] = [{"ef": pa["ef"], "reduced_over_fp32": round(pb["qps"] / pa["qps"], 3),
+                         "recall_fp32": pa["recall"], "recall_reduced": pb["recall"]}
+                        for pa, pb in zip(per["fp32"]["points"], per["reduced"]["points"]) if pa["ef"] == pb["ef"]]
         result["k"][str(k)] = per
     dev.set_option("storage", storage)
     result["stored_distance_error_vs_fp64"] = stored_distance_error(dev, X, Q, storage, metric)
