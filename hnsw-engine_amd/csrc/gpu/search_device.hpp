@@ -1282,24 +1282,38 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
 /// bitmap that excludes only a few deleted nodes (one launch, or a mixed one).
 enum FilterMode : int { kPlain = 0, kFiltered = 1, kPerQuery = 2, kDeleted = 3, kPerQueryDeleted = 4 };
 
-/// One query. In a mixed launch (kPerQuery, kPerQueryDeleted) the kernel is
-/// built for the largest capacity class of the batch and a query with a
-/// smaller ef runs the code of its own class.
+/// One query of a uniform launch (one filter mode and capacity class).
 template <int ER, int EC, int PC, int FM, int ST, int RT>
 __device__ Handoff run_query(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s, const WaveLds& w,
                              Visited& vis, uint32_t qi, uint64_t* runs) {
-  if constexpr ((FM == kPerQuery || FM == kPerQueryDeleted) && ER > 1) {
-    if (a.ef <= 32u * ER) return run_query<ER / 2, EC / 2, PC, FM, ST, RT>(ix, a, s, w, vis, qi, runs);
-  }
   if constexpr (FM == kPlain) {
     return plain_query<ER, PC, ST, false, RT>(ix, a, s, w, vis, qi);
   } else if constexpr (FM == kFiltered) {
     return filtered_query<ER, EC, PC, ST, RT>(ix, a, s, w, vis, qi, runs);
-  } else if constexpr (FM == kDeleted || FM == kPerQueryDeleted) {
-    return plain_query<ER, PC, ST, true, RT>(ix, a, s, w, vis, qi);
   } else {
-    if (a.elig == nullptr) return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
-    return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi, runs);
+    return plain_query<ER, PC, ST, true, RT>(ix, a, s, w, vis, qi);
+  }
+}
+
+/// One query of a mixed launch (kPerQuery, kPerQueryDeleted): the kernel is
+/// built for the largest capacity class of the batch and a query with a
+/// smaller ef runs the code of its own class. Every
+/// class is inlined into the kernel: a call would take the index view,
+/// query arguments, scratch and LDS views and visited state by address, which
+/// keeps them in scratch memory, read from the search loops of every class.
+template <int ER, int EC, int PC, int FM, int ST, int RT>
+__device__ __forceinline__ Handoff run_mixed(const DevIndexView& ix, const QueryArgs& a, const SearchScratch& s,
+                                             const WaveLds& w, Visited& vis, uint32_t qi, uint64_t* runs) {
+  if constexpr (ER > 1) {
+    if (a.ef <= 32u * ER) return run_mixed<ER / 2, EC / 2, PC, FM, ST, RT>(ix, a, s, w, vis, qi, runs);
+  }
+  if constexpr (FM == kPerQueryDeleted) {
+    [[clang::always_inline]] return plain_query<ER, PC, ST, true, RT>(ix, a, s, w, vis, qi);
+  } else {
+    if (a.elig == nullptr) {
+      [[clang::always_inline]] return plain_query<ER, PC, ST>(ix, a, s, w, vis, qi);
+    }
+    [[clang::always_inline]] return filtered_query<ER, EC, PC, ST>(ix, a, s, w, vis, qi, runs);
   }
 }
 
@@ -1332,7 +1346,13 @@ __global__ void __launch_bounds__(kWavesPerBlock * 64) fast_search(DevIndexView 
     if (!r.force_spill) {
       const QueryArgs a = query_args(r, qi);
       // A task whose ef exceeds the register capacity goes to the general kernel.
-      if (a.ef <= 64u * ER) h = run_query<ER, EC, PC, FM, ST, RT>(ix, a, s, w, vis, qi, runs);
+      if (a.ef <= 64u * ER) {
+        if constexpr (FM == kPerQuery || FM == kPerQueryDeleted) {
+          h = run_mixed<ER, EC, PC, FM, ST, RT>(ix, a, s, w, vis, qi, runs);
+        } else {
+          h = run_query<ER, EC, PC, FM, ST, RT>(ix, a, s, w, vis, qi, runs);
+        }
+      }
     }
     if (h != Handoff::kNone && lane == 0) {
       const uint32_t slot = atomicAdd(&s.counters[1], 1u);
