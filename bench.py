@@ -17,6 +17,12 @@ recall). The roofline figure uses the fast kernel's own event-timed duration
 and the algorithmic bytes per query counted by the kernel itself (rows,
 layer-0 lists and upper lists read, query in, results out).
 
+A second sweep runs the same steps alternating over two streams (two contexts
+of the handle, two output buffers), so that one launch's tail (its last queries
+running on a part-empty GPU) overlaps the next launch's start; it is reported
+as `pipelined` beside `value`, which stays one launch per step, and its rows
+must equal the serial run's.
+
 The CPU baseline is the reference's search_batch (compiled from its sources
 into oracle/_ref) on the same index file with all host cores, over the same
 queries and ef sweep; its rows are compared with the GPU's row by row. The
@@ -490,6 +496,7 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (counter calibration) and exit")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop section")
+    ap.add_argument("--no-pipelined", action="store_true", help="skip the two-stream pipelined sweep")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the 1M x 100 cosine bf16 and 200k x 960 fp8 configurations")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
@@ -657,6 +664,45 @@ def main() -> int:
             f"bytes/q={bytes_q:,.0f} achieved={gbs:,.0f} GB/s dist/q={n_dist:.0f} general={overflow} "
             f"hbm-visited={spilled}")
 
+    # Pipelined: the same steps alternating over two streams (two contexts of the handle, two
+    # output buffers), so that a launch's tail, when its last queries run on a part-empty GPU,
+    # overlaps the next launch's start. Each step is still one complete search of the batch.
+    pipelined = None
+    if not args.profile and not args.no_pipelined:
+        stream2 = hnswgpu.Stream()
+        d_out2 = hnswgpu.DeviceBuffer(args.nq * k * hnswgpu.NEIGHBOR.itemsize)
+        pp = []
+        for ef in efs:
+            def step2(i, ef=ef):
+                st, out = (stream, d_out) if i % 2 == 0 else (stream2, d_out2)
+                dev.search_device(d_q.ptr, args.nq, k, ef, out.ptr, stream=st.handle)
+            for i in range(max(2, args.warmup)):
+                step2(i)
+            stream.synchronize()
+            stream2.synchronize()
+            t0 = time.perf_counter()
+            for i in range(args.steps):
+                step2(i)
+            stream.synchronize()
+            stream2.synchronize()
+            dt = (time.perf_counter() - t0) / args.steps
+            rows2 = d_out2.download(host_rows).reshape(args.nq, k)
+            same = bool(np.array_equal(rows2["label"], gpu_rows[ef]["label"]) and
+                        np.array_equal(rows2["distance"].view(np.uint32), gpu_rows[ef]["distance"].view(np.uint32)))
+            rec = per_ef[ef]["recall"]
+            pp.append((ef, rec, args.nq / dt))
+            log(f"gpu pipelined ef={ef} qps={args.nq / dt:,.0f} step={dt * 1e3:.3f}ms rows equal the serial run: {same}")
+            if not same:
+                raise SystemExit("pipelined rows differ from the serial run")
+        v2 = qps_at(pp)
+        pipelined = {"streams": 2, "value": round(v2, 1) if math.isfinite(v2) else None,
+                     "over_serial": round(v2 / qps_at(points), 3) if math.isfinite(v2) else None,
+                     # Algorithmic bytes of a step over the pipelined step time, against 8 TB/s.
+                     "points": [{"ef": e, "recall": r, "qps": round(q, 1),
+                                 "hbm_frac": round(per_ef[e]["bytes_per_query"] * q / 1e9 / HBM_PEAK_GBS, 4)}
+                                for e, r, q in pp]}
+        del d_out2, stream2
+
     if args.profile:
         print(json.dumps({"profile": True, "points": list(per_ef.values())}), flush=True)
         return 0
@@ -764,6 +810,7 @@ def main() -> int:
                      "kernel": "fast_search", "at_ef": op["ef"], "kernel_ms": op["kernel_ms"],
                      "bytes_per_query": op["bytes_per_query"]},
         "cpu_baseline": cpu,
+        "pipelined": pipelined,
         "parity": parity,
         "build": build,
         "brute_force": brute,

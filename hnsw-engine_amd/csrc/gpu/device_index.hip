@@ -233,13 +233,24 @@ uint32_t side_list_for(const hnswgpu_index& h, double allowed) {
   return allowed * kept * 16.0 >= 15.0 ? 1u : 0u;
 }
 
-CallCtx* acquire_ctx(hnswgpu_index& h) {
+CallCtx* acquire_ctx(hnswgpu_index& h, hipStream_t st) {
   std::unique_lock<std::mutex> lk(h.pool_mu);
   h.pool_cv.wait(lk, [&] { return !h.idle.empty() || h.ctxs.size() < h.max_ctxs; });
-  if (!h.idle.empty()) {
-    CallCtx* c = h.idle.back();
-    h.idle.pop_back();
+  auto take = [&](size_t i) {
+    CallCtx* c = h.idle[i];
+    h.idle.erase(h.idle.begin() + static_cast<std::ptrdiff_t>(i));
     return c;
+  };
+  if (st != nullptr) {
+    for (size_t i = h.idle.size(); i-- > 0;) {
+      if (h.idle[i]->used && h.idle[i]->last_stream == st) return take(i);
+    }
+    for (size_t i = h.idle.size(); i-- > 0;) {
+      if (!h.idle[i]->used || hipEventQuery(h.idle[i]->done) == hipSuccess) return take(i);
+    }
+    if (h.ctxs.size() >= h.max_ctxs) return take(h.idle.size() - 1);
+  } else if (!h.idle.empty()) {
+    return take(h.idle.size() - 1);
   }
   auto c = std::make_unique<CallCtx>();
   check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -602,8 +613,8 @@ void device_search_dev(hnswgpu_index* h, const float* d_qs, uint64_t nq, uint64_
   if (d_qs == nullptr || d_out == nullptr) throw std::invalid_argument("search_batch: null pointer");
   std::shared_lock<std::shared_mutex> lk(h->mu);
   if (h->view.count == 0) throw std::invalid_argument("search_batch: empty index on the device path");
-  dev::CtxLease lease(*h);
   const auto st = static_cast<hipStream_t>(stream);
+  dev::CtxLease lease(*h, st);
   dev::ctx_begin(*lease.c, st);
   search_device(*h, *lease.c, d_qs, nullptr, nq, k, ef, d_words, filter_size, by_id, d_out, d_counts, st);
   dev::ctx_end(*lease.c, st);

@@ -231,8 +231,14 @@ uint32_t deleted_side_list(const hnswgpu_index& h);
 /// unknown): the side list when at least 15 nodes in 16 are eligible.
 uint32_t side_list_for(const hnswgpu_index& h, double allowed);
 /// Take a call context from the pool (waits while max_ctxs are in use) and
-/// give it back; the one given back last is the handle's most recent.
-CallCtx* acquire_ctx(hnswgpu_index& h);
+/// give it back; the one given back last is the handle's most recent. A call
+/// that enqueues on the caller's stream `st` (non-null) is given, in this
+/// order: an idle context last used on st (stream order already follows its
+/// work), one whose work has finished, a new one while fewer than max_ctxs
+/// exist, else the most recent (st then waits for its work). So calls on
+/// different streams overlap on the device, and calls on one stream reuse one
+/// context.
+CallCtx* acquire_ctx(hnswgpu_index& h, hipStream_t st = nullptr);
 void release_ctx(hnswgpu_index& h, CallCtx* c);
 /// Before enqueuing on st: order st after the context's earlier work when
 /// that ran on another stream. After: record the context's event on st.
@@ -242,7 +248,7 @@ void ctx_end(CallCtx& c, hipStream_t st);
 struct CtxLease {
   hnswgpu_index& h;
   CallCtx* c;
-  explicit CtxLease(hnswgpu_index& hh) : h(hh), c(acquire_ctx(hh)) {}
+  explicit CtxLease(hnswgpu_index& hh, hipStream_t st = nullptr) : h(hh), c(acquire_ctx(hh, st)) {}
   ~CtxLease() { release_ctx(h, c); }
   CtxLease(const CtxLease&) = delete;
   CtxLease& operator=(const CtxLease&) = delete;

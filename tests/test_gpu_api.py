@@ -142,6 +142,41 @@ def test_concurrent_search_batch_on_one_index(lib):
     assert 1 <= dev.get_option("contexts") <= dev.get_option("max_contexts")
 
 
+def test_device_searches_on_two_streams_overlap_safely(lib):
+    """Asynchronous device-buffer searches alternating over two streams take
+    two contexts of the handle (their launches may overlap on the device) and
+    each returns the rows of the same search alone; calls on one stream keep
+    reusing one context."""
+    import workloads
+
+    X = workloads.mixture_128(20000, seed=25)
+    Q = workloads.mixture_128(3000, seed=1025)
+    ix = lib.Index(128, "l2")
+    ix.add(X, num_threads=0)
+    dev = ix.device()
+    efs = [16, 64, 128, 256, 100, 40]
+    want = [ix.search_rows(Q, 10, ef) for ef in efs]
+    before = dev.get_option("contexts")
+    d_q = lib.DeviceBuffer.of(np.ascontiguousarray(Q, np.float32))
+    streams = [lib.Stream(), lib.Stream()]
+    outs = [lib.DeviceBuffer(Q.shape[0] * 10 * lib.NEIGHBOR.itemsize) for _ in efs]
+    for rep in range(2):
+        for i, ef in enumerate(efs):
+            dev.search_device(d_q.ptr, Q.shape[0], 10, ef, outs[i].ptr, stream=streams[i % 2].handle)
+        for s in streams:
+            s.synchronize()
+        for i in range(len(efs)):
+            rows = outs[i].download(np.empty(Q.shape[0] * 10, lib.NEIGHBOR)).reshape(Q.shape[0], 10)
+            assert rows_equal(rows, want[i]), (rep, efs[i])
+    assert before <= dev.get_option("contexts") <= min(before + 2, dev.get_option("max_contexts"))
+    n = dev.get_option("contexts")
+    one = lib.Stream()
+    for ef in efs:
+        dev.search_device(d_q.ptr, Q.shape[0], 10, ef, outs[0].ptr, stream=one.handle)
+    one.synchronize()
+    assert dev.get_option("contexts") <= n + 1
+
+
 def test_mmap_loaded_index_equals_golden_rows(golden, lib):
     for name in ("l2_d32", "cos_d100", "ties_d8"):
         ix = lib.Index.load(GOLDEN / f"{name}_avx512.idx", mmap=True)
