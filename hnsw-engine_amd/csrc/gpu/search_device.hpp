@@ -377,11 +377,24 @@ struct Hash16 {
   }
 };
 
+// Load of the LDS visited hash (in eighths) at which a query's visited set
+// moves on to its HBM table. Unfiltered searches fit the LDS table at 3/4 up
+// to ef 128 and mostly at 256 (7/8 measured 2-4 % slower from longer probes,
+// 5/8 6 % slower at ef 256 from more spills). A sparse filter's search visits
+// several times as many nodes and spills anyway: moving earlier, at 5/8,
+// spares it the long probes (10 % filter at ef 256: 6 % faster; 4/8 lost).
+#ifndef HNSWGPU_LDS_LOAD_EIGHTHS
+#define HNSWGPU_LDS_LOAD_EIGHTHS 6
+#endif
+#ifndef HNSWGPU_FILTERED_LDS_EIGHTHS
+#define HNSWGPU_FILTERED_LDS_EIGHTHS 5
+#endif
+
 /// Exact visited set of one query: an open-addressing hash in LDS (16-bit
-/// entries when ids fit, else 32-bit) and, once that reaches 3/4 load, a
-/// second one in HBM (one table per resident wavefront, emptied after a query
-/// that used it). After the switch new ids go to the HBM table; lookups check
-/// LDS first.
+/// entries when ids fit, else 32-bit) and, once that reaches 3/4 load (5/8 in
+/// the filtered search), a second one in HBM (one table per resident
+/// wavefront, emptied after a query that used it). After the switch new ids
+/// go to the HBM table; lookups check LDS first.
 struct Visited {
   uint32_t* lds;
   uint32_t lds_log2;  // LDS words
@@ -401,11 +414,12 @@ struct Visited {
     const uint32_t e = compact ? 0u : kEmptySlot;
     for (uint32_t i = static_cast<uint32_t>(lane_id()); i < (1u << lds_log2); i += 64) lds[i] = e;
   }
-  /// Room for `more` new ids? Switches to the HBM table when the LDS one is
-  /// full; false when both are.
+  /// Room for `more` new ids? Switches to the HBM table when the LDS one
+  /// reaches EIGHTHS / 8 load; false when both are full.
+  template <uint32_t EIGHTHS = HNSWGPU_LDS_LOAD_EIGHTHS>
   __device__ __forceinline__ bool reserve(uint32_t more) {
     if (!spilled) {
-      if (in_lds + more <= (lds_slots() >> 2) * 3) return true;
+      if (in_lds + more <= (lds_slots() >> 3) * EIGHTHS) return true;
       if (hbm == nullptr) return false;
       spilled = true;
     }
@@ -1219,7 +1233,7 @@ __device__ Handoff filtered_query(const DevIndexView& ix, const QueryArgs& a, co
     if (sR >= ef && key_ord(c) > tau) break;
     C.pop_front();
     --sC;
-    if (!vis.reserve(ix.M0)) {
+    if (!vis.reserve<HNSWGPU_FILTERED_LDS_EIGHTHS>(ix.M0)) {
       result = Handoff::kVisited;
       break;
     }
