@@ -113,7 +113,11 @@ int hnswgpu_search_batch(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_
  * include/hnsw/index.hpp:31-36). Each call in flight takes a context of its
  * own (scratch, staging, stream) from the handle's pool (option
  * "max_contexts", default 4; further callers wait for one), so their launches
- * overlap on the device. Uploads, deletions, options and tuning are exclusive. */
+ * overlap on the device. An asynchronous device call (hnswgpu_search_batch_device)
+ * on a non-null stream takes the context last used on that stream, else one
+ * whose work has finished, else a new one: calls on different streams overlap
+ * (alternating two streams hides each launch's tail), calls on one stream
+ * reuse one context. Uploads, deletions, options and tuning are exclusive. */
 
 /* Replaces: Index::search_batch with a Filter that is not a BitsetFilter
  * (include/hnsw/filter.hpp:12-30: FunctionFilter or any subclass), which the
