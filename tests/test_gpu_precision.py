@@ -83,28 +83,35 @@ def test_fp8_rows_are_ocp_e4m3_with_scale(golden, lib):
 
 @pytest.mark.parametrize("storage", [1, 2])
 def test_reduced_traversal_returns_exact_distances(golden, lib, oracle, storage):
+    """Dimension 128 (the fixed row width): ef 64 (insert re-rank) and ef 400
+    (the largest capacity class, re-ranked in place and sorted) return the
+    exact fp32 distances of their ids, sorted by (distance, id), at the recall
+    of the fp32 traversal."""
     import workloads
 
     X = workloads.mixture_128(20000, seed=9)
     Q = workloads.mixture_128(300, seed=1009)
     ix = lib.Index(128, "l2")
     ix.add(X)
-    exact = ix.search_rows(Q, 10, 64)
     dev = ix.device()
-    dev.set_option("storage", storage)
-    got = ix.search_rows(Q, 10, 64)
-    dev.set_option("storage", 0)
-    lab = got["label"].astype(np.int64)
-    # Distances are the exact fp32 distances (AVX-512 order) of the returned ids.
-    want = np.array([[oracle.l2(Q[i], X[j]) for j in lab[i]] for i in range(Q.shape[0])], np.float32)
-    assert np.array_equal(got["distance"].view(np.uint32), want.view(np.uint32))
-    key = got["distance"].astype(np.float64) * 2 ** 32 + lab
-    assert np.all(np.diff(key, axis=1) > 0)
-    d64 = ((Q[:, None, :].astype(np.float64) - X[None, :, :]) ** 2).sum(-1)
-    truth = np.argsort(d64, axis=1, kind="stable")[:, :10]
-    r_exact = workloads.recall(exact["label"], truth, 10)
-    r_low = workloads.recall(got["label"], truth, 10)
-    assert r_low >= r_exact - 0.02, (r_low, r_exact)
+    # Exact in float64: the data are integers below 256.
+    q64, x64 = Q.astype(np.float64), X.astype(np.float64)
+    d64 = (q64 ** 2).sum(1)[:, None] + (x64 ** 2).sum(1)[None, :] - 2.0 * q64 @ x64.T
+    for k, ef in ((10, 64), (10, 400), (100, 400)):
+        exact = ix.search_rows(Q, k, ef)
+        dev.set_option("storage", storage)
+        got = ix.search_rows(Q, k, ef)
+        dev.set_option("storage", 0)
+        lab = got["label"].astype(np.int64)
+        # Distances are the exact fp32 distances (AVX-512 order) of the returned ids.
+        want = np.array([[oracle.l2(Q[i], X[j]) for j in lab[i]] for i in range(Q.shape[0])], np.float32)
+        assert np.array_equal(got["distance"].view(np.uint32), want.view(np.uint32)), (k, ef)
+        key = got["distance"].astype(np.float64) * 2 ** 32 + lab
+        assert np.all(np.diff(key, axis=1) > 0), (k, ef)
+        truth = np.argsort(d64, axis=1, kind="stable")[:, :k]
+        r_exact = workloads.recall(exact["label"], truth, k)
+        r_low = workloads.recall(got["label"], truth, k)
+        assert r_low >= r_exact - 0.02, (k, ef, r_low, r_exact)
 
 
 def test_reduced_cosine_and_filters(golden, lib):
