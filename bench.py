@@ -144,8 +144,7 @@ def measured_traffic(ef: int, args, kernel_ms: float) -> dict:
             "traffic_profile_kernel_ms": prof_ms}
 
 
-def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3, max_cost: int = 0, heavy_cost: int = 0,
-                  heavy_hash_log2: int = 0) -> dict:
+def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3, max_cost: int = 0, heavy_cost: int = 0) -> dict:
     """The serving loop under the mixed load of a production index: requests
     of log-uniform batch size 1..512, k in {1, 10, 100}, ef in {32..512},
     30 % with a 10 %-density BitsetFilter, half traversing the BF16 copy, 1 %
@@ -181,8 +180,7 @@ def serving_bench(ix, Q: np.ndarray, seconds: float, slots: int = 3, max_cost: i
                 "launches": int(st["launches"]), "mean_queries_per_launch": round(st["mean_batch"], 1),
                 "graph_launches": int(st["graph_launches"])}
 
-    srv = hnswgpu.Server(dev, slots=slots, max_batch=4096, storages=(1,), max_cost=max_cost, heavy_cost=heavy_cost,
-                         heavy_hash_log2=heavy_hash_log2)
+    srv = hnswgpu.Server(dev, slots=slots, max_batch=4096, storages=(1,), max_cost=max_cost, heavy_cost=heavy_cost)
     # Warm-up: every slot captures one graph per (storage, ef class, kernel)
     # on first use, the kernel being the mixed one when a launch holds a
     # filtered request and the deletions side-list one when it holds none.
@@ -509,8 +507,6 @@ def main() -> int:
     ap.add_argument("--serving-max-cost", type=int, default=0, help="cost units per launch (0: the server's default)")
     ap.add_argument("--serving-heavy-cost", type=int, default=0,
                     help="query cost from which requests form launches of their own (0: the server's default)")
-    ap.add_argument("--serving-heavy-hash", type=int, default=0,
-                    help="LDS visited-hash words (log2) of heavy launches (0: the handle's choice)")
     ap.add_argument("--serving-only", action="store_true", help="only the serving section, one line")
     ap.add_argument("--profile-brute", action="store_true",
                     help="profiling run of the brute force only (n x 128 data, nq queries, k), one line")
@@ -578,8 +574,7 @@ def main() -> int:
 
     if args.serving_only:
         print(json.dumps({"serving": serving_bench(ix, Q, args.serving_seconds, args.serving_slots,
-                                                   args.serving_max_cost, args.serving_heavy_cost,
-                                                   args.serving_heavy_hash)}), flush=True)
+                                                   args.serving_max_cost, args.serving_heavy_cost)}), flush=True)
         return 0
 
     if args.deleted > 0:
@@ -784,8 +779,7 @@ def main() -> int:
 
     # Last: it soft-deletes 1 % of the base.
     serving = None if args.no_serving else serving_bench(ix, Q, args.serving_seconds, args.serving_slots,
-                                                         args.serving_max_cost, args.serving_heavy_cost,
-                                                         args.serving_heavy_hash)
+                                                         args.serving_max_cost, args.serving_heavy_cost)
     del ix, dev
     precision = {}
     if not args.no_precision:

@@ -343,7 +343,7 @@ uint32_t ef_class_index(uint32_t ef) {
   return ef <= 64 ? 0u : ef <= 128 ? 1u : ef <= 256 ? 2u : 3u;
 }
 
-void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq, uint32_t hash_log2) {
+void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq) {
   const uint32_t ef = std::min<uint32_t>(r.ef, gpu::kFastMaxEf);
   uint32_t waves_per_cu = h.waves_per_cu;
   r.hash_log2 = h.hash_log2 != 0 ? h.hash_log2 : auto_hash_log2(ef, h.view.count, h.wide_hash != 0);
@@ -362,7 +362,6 @@ void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq, ui
     r.row_trip = t.row_trip;
     r.min_class = t.min_class;
   }
-  if (hash_log2 != 0) r.hash_log2 = hash_log2;  // the caller's launch shape (a server's heavy launches)
   const size_t per_wave = static_cast<size_t>(gpu::lds_words_per_wave(r.hash_log2, h.view.padded)) * 4u;
   r.waves_per_block = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / per_wave)));
   // A persistent grid of the resident wavefronts (each owns an HBM visited table).

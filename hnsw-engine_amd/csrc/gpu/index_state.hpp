@@ -216,7 +216,7 @@ void reupload(hnswgpu_index& h, const hnswgpu_index_view& v);
 void build_reduced(hnswgpu_index& h, int storage);
 /// Launch shape for r.ef and nq queries: hash size, wavefronts per block and a
 /// persistent grid of the resident wavefronts.
-void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq, uint32_t hash_log2 = 0);
+void plan_request(const hnswgpu_index& h, gpu::SearchRequest& r, uint32_t nq);
 /// Scratch for a launch planned by plan_request with room for nq queries.
 gpu::SearchScratch bind_scratch(hnswgpu_index& h, Scratch& sc, const gpu::SearchRequest& r, uint32_t nq,
                                 uint32_t spill_waves, hipStream_t st);

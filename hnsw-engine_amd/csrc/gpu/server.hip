@@ -115,7 +115,7 @@ struct Slot {
   hnswamd::dev::Scratch sc;
   hnswamd::dev::PinnedBuf stage, result;
   hnswamd::dev::DevBuf in, out, elig;
-  std::map<std::tuple<int, uint32_t, int, bool>, Graph> graphs;  // (storage, ef class, deletions-only kernel, heavy)
+  std::map<std::tuple<int, uint32_t, int>, Graph> graphs;  // (storage, ef class, deletions-only kernel)
   std::vector<Piece> pieces;
   bool heavy = false;     // the launch holds heavy requests
   bool launched = false;  // a launch is in flight (its completion thread owns the slot)
@@ -349,7 +349,7 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
   r.force_spill = h.force_spill;
   // Deletions and no filter in the whole launch: the side-list kernel.
   r.deleted_only = deletions && filters.empty() ? static_cast<int>(side_list) : 0;
-  hnswamd::dev::plan_request(h, r, r.nq, slot.heavy ? c.heavy_hash_log2 : 0u);
+  hnswamd::dev::plan_request(h, r, r.nq);
   hnswamd::gpu::SearchScratch s = hnswamd::dev::bind_scratch(h, slot.sc, r, r.nq, c.spill_waves, slot.stream);
   auto* d_descs = reinterpret_cast<const FilterDesc*>(dbase + srv.descs_off);
   auto* d_elig = static_cast<uint32_t*>(slot.elig.p);
@@ -364,7 +364,7 @@ void launch(hnswgpu_server& srv, Slot& slot, int storage, uint64_t max_ef,
     check(hnswamd::gpu::launch_search(ix, r, s, slot.stream), "search kernel");
     check(hipMemcpyAsync(slot.result.p, slot.out.p, out_n, hipMemcpyDeviceToHost, slot.stream), "hipMemcpy out");
   };
-  const auto key = std::make_tuple(storage, cls, r.deleted_only, slot.heavy);
+  const auto key = std::make_tuple(storage, cls, r.deleted_only);
   std::vector<uint64_t> shape = shape_words(r, s, ix, del_bits);
   auto it = slot.graphs.find(key);
   if (it != slot.graphs.end() && it->second.shape != shape) {
@@ -552,9 +552,6 @@ int hnswgpu_server_create(hnswgpu_index* h, const hnswgpu_server_config* config,
     if (c.heavy_cost == 0) c.heavy_cost = 1024;
     if (c.slots > 16 || c.max_batch > (1u << 20) || c.max_filters > 256) {
       throw std::invalid_argument("server_create: slots <= 16, max_batch <= 2^20, max_filters <= 256");
-    }
-    if (c.heavy_hash_log2 != 0 && (c.heavy_hash_log2 < 10 || c.heavy_hash_log2 > 13)) {
-      throw std::invalid_argument("server_create: heavy_hash_log2 must be 0 or 10..13");
     }
     {
       std::unique_lock<std::shared_mutex> lk(h->mu);

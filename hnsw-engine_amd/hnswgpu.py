@@ -279,7 +279,7 @@ class ServerConfig(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("max_batch", C.c_uint32), ("max_rows", C.c_uint64),
                 ("max_filters", C.c_uint32), ("use_graphs", C.c_uint32), ("max_filter_labels", C.c_uint64),
                 ("hbm_budget", C.c_uint64), ("spill_waves", C.c_uint32), ("storages", C.c_uint32),
-                ("max_cost", C.c_uint64), ("heavy_cost", C.c_uint64), ("heavy_hash_log2", C.c_uint32)]
+                ("max_cost", C.c_uint64), ("heavy_cost", C.c_uint64)]
 
 
 class BuildStats(C.Structure):
@@ -305,11 +305,9 @@ class Server:
 
     def __init__(self, device: "DeviceIndex", slots: int = 2, max_batch: int = 4096, max_rows: int = 0,
                  max_filters: int = 8, graphs: bool = True, max_filter_labels: int = 0, hbm_budget: int = 0,
-                 spill_waves: int = 0, storages=(), max_cost: int = 0, heavy_cost: int = 0,
-                 heavy_hash_log2: int = 0):
+                 spill_waves: int = 0, storages=(), max_cost: int = 0, heavy_cost: int = 0):
         cfg = ServerConfig(slots, max_batch, max_rows, max_filters, 1 if graphs else 0, max_filter_labels,
-                           hbm_budget, spill_waves, sum(1 << int(st) for st in storages), max_cost, heavy_cost,
-                           heavy_hash_log2)
+                           hbm_budget, spill_waves, sum(1 << int(st) for st in storages), max_cost, heavy_cost)
         h = C.c_void_p()
         _check(_lib.hnswgpu_server_create(device._h, C.byref(cfg), C.byref(h)))
         self._h, self._device, self._live = h, device, {}

@@ -221,9 +221,6 @@ typedef struct hnswgpu_server_config {
                                   10 %-density filter at ef 128) form launches of their own, which
                                   hold at most all but one slot, so that cheap requests never wait
                                   for them */
-  uint32_t heavy_hash_log2;    /* LDS visited-hash words (log2, 10..13) of heavy launches; 0 = the
-                                  handle's choice. A larger hash keeps a sparse filter's visited set
-                                  out of HBM (faster queries) at fewer resident wavefronts */
 } hnswgpu_server_config;
 
 /* One request: search_batch's arguments on host buffers, which must stay valid

@@ -88,31 +88,25 @@ def test_each_result_same_alone_and_in_the_mix(lib, served):
 
 def test_wait_latency_and_heavy_classes(lib, served):
     """Per-request latency readout, and heavy requests (expensive queries:
-    filtered at high ef) in launches of their own, also with a larger LDS
-    visited hash of their own (heavy_hash_log2): results unchanged, and the
+    filtered at high ef) in launches of their own: results unchanged, and the
     latencies are the ones the statistics count."""
     ix, dev, X, Q, filters = served
     rng = np.random.default_rng(12)
     reqs = make_requests(rng, Q, 40, filters)
-    for heavy_hash in (0, 13, 12):
-        srv = lib.Server(dev, slots=3, max_batch=1024, storages=(1,), heavy_cost=256, max_cost=1 << 16,
-                         heavy_hash_log2=heavy_hash)
-        try:
-            tickets = [srv.submit(r["q"], r["k"], r["ef"], None if r["fid"] < 0 else filters[r["fid"]],
-                                  storage=r["storage"]) for r in reqs]
-            outs = [srv._live[t][2] for t in tickets]
-            lat = [srv.wait_latency(t) for t in tickets]
-            st = srv.stats()
-        finally:
-            srv.close()
-        assert all(l > 0 for l in lat), heavy_hash
-        assert abs(max(lat) - st["max_ms"]) < 1e-6 and st["requests"] == len(reqs), heavy_hash
-        for r, out in zip(reqs, outs):
-            want, _ = alone(dev, r, filters)
-            assert rows_equal(out.reshape(r["q"].shape[0], -1), want), (heavy_hash, r["k"], r["ef"], r["fid"],
-                                                                         r["storage"])
-    with pytest.raises(ValueError):
-        lib.Server(dev, slots=2, heavy_hash_log2=9)
+    srv = lib.Server(dev, slots=3, max_batch=1024, storages=(1,), heavy_cost=256, max_cost=1 << 16)
+    try:
+        tickets = [srv.submit(r["q"], r["k"], r["ef"], None if r["fid"] < 0 else filters[r["fid"]],
+                              storage=r["storage"]) for r in reqs]
+        outs = [srv._live[t][2] for t in tickets]
+        lat = [srv.wait_latency(t) for t in tickets]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert all(l > 0 for l in lat)
+    assert abs(max(lat) - st["max_ms"]) < 1e-6 and st["requests"] == len(reqs)
+    for r, out in zip(reqs, outs):
+        want, _ = alone(dev, r, filters)
+        assert rows_equal(out.reshape(r["q"].shape[0], -1), want), (r["k"], r["ef"], r["fid"], r["storage"])
 
 
 def test_graph_replays_equal_eager_launches(lib, served):
