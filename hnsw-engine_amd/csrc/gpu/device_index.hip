@@ -270,7 +270,9 @@ void release_ctx(hnswgpu_index& h, CallCtx* c) {
 }
 
 void ctx_begin(CallCtx& c, hipStream_t st) {
-  if (c.used && c.last_stream != st) check(hipStreamWaitEvent(st, c.done, 0), "hipStreamWaitEvent");
+  // Also on the stream last used: a caller may have destroyed that stream with
+  // work in flight and been given a new one with the same handle.
+  if (c.used) check(hipStreamWaitEvent(st, c.done, 0), "hipStreamWaitEvent");
 }
 
 void ctx_end(CallCtx& c, hipStream_t st) {

@@ -233,15 +233,14 @@ uint32_t side_list_for(const hnswgpu_index& h, double allowed);
 /// Take a call context from the pool (waits while max_ctxs are in use) and
 /// give it back; the one given back last is the handle's most recent. A call
 /// that enqueues on the caller's stream `st` (non-null) is given, in this
-/// order: an idle context last used on st (stream order already follows its
-/// work), one whose work has finished, a new one while fewer than max_ctxs
-/// exist, else the most recent (st then waits for its work). So calls on
-/// different streams overlap on the device, and calls on one stream reuse one
-/// context.
+/// order: an idle context last used on st, one whose work has finished, a
+/// new one while fewer than max_ctxs exist, else the most recent (st then
+/// waits for its work). So calls on different streams overlap on the device,
+/// and calls on one stream reuse one context.
 CallCtx* acquire_ctx(hnswgpu_index& h, hipStream_t st = nullptr);
 void release_ctx(hnswgpu_index& h, CallCtx* c);
-/// Before enqueuing on st: order st after the context's earlier work when
-/// that ran on another stream. After: record the context's event on st.
+/// Before enqueuing on st: order st after the context's earlier work. After:
+/// record the context's event on st.
 void ctx_begin(CallCtx& c, hipStream_t st);
 void ctx_end(CallCtx& c, hipStream_t st);
 /// A context held for the scope of one call.
