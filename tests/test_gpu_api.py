@@ -145,24 +145,31 @@ def test_concurrent_search_batch_on_one_index(lib):
 def test_device_searches_on_two_streams_overlap_safely(lib):
     """Asynchronous device-buffer searches alternating over two streams take
     two contexts of the handle (their launches may overlap on the device) and
-    each returns the rows of the same search alone; calls on one stream keep
-    reusing one context."""
+    each returns the rows of the same search alone, with deletions and, for
+    some, a label filter; calls on one stream keep reusing one context."""
     import workloads
 
     X = workloads.mixture_128(20000, seed=25)
     Q = workloads.mixture_128(3000, seed=1025)
     ix = lib.Index(128, "l2")
     ix.add(X, num_threads=0)
+    for label in range(3, 20000, 101):
+        ix.mark_deleted(label)
     dev = ix.device()
     efs = [16, 64, 128, 256, 100, 40]
-    want = [ix.search_rows(Q, 10, ef) for ef in efs]
+    # Every third search with a 30 % label filter (eligibility bitmaps in the context's scratch).
+    fw, fsize = lib.filter_words(np.random.default_rng(6).random(20000) < 0.3)
+    d_f = lib.DeviceBuffer.of(fw)
+    filt = [i % 3 == 2 for i in range(len(efs))]
+    want = [ix.search_rows(Q, 10, ef, filter=(fw, fsize) if f else None) for ef, f in zip(efs, filt)]
     before = dev.get_option("contexts")
     d_q = lib.DeviceBuffer.of(np.ascontiguousarray(Q, np.float32))
     streams = [lib.Stream(), lib.Stream()]
     outs = [lib.DeviceBuffer(Q.shape[0] * 10 * lib.NEIGHBOR.itemsize) for _ in efs]
     for rep in range(2):
         for i, ef in enumerate(efs):
-            dev.search_device(d_q.ptr, Q.shape[0], 10, ef, outs[i].ptr, stream=streams[i % 2].handle)
+            dev.search_device(d_q.ptr, Q.shape[0], 10, ef, outs[i].ptr, stream=streams[i % 2].handle,
+                              d_filter=d_f.ptr if filt[i] else 0, filter_size=fsize if filt[i] else 0)
         for s in streams:
             s.synchronize()
         for i in range(len(efs)):
