@@ -463,6 +463,17 @@ class DeviceIndex:
                                                         d_out, d_counts or None, stream or None))
 
 
+__version__ = "0.1.0"
+
+
+def simd_isa() -> str:
+    """Name of the reference kernel whose fp32 summation order every distance
+    of this library follows (the reference's simd_isa, python/bindings.cpp:117-119):
+    always "avx512", in the GPU kernels and in the host builder alike, whatever
+    the host CPU supports."""
+    return "avx512"
+
+
 class Index:
     """hnsw::Index over the C ABI."""
 
@@ -605,6 +616,11 @@ class Index:
     max_level = property(lambda self: int(self._info()[7]))
     read_only = property(lambda self: bool(self._info()[8]))
     memory_usage = property(lambda self: int(self._info()[9]))
+
+    def __repr__(self) -> str:
+        # The reference's text (python/bindings.cpp:228-233), under this module's name.
+        return (f"<hnswgpu.Index dim={self.dim} metric={self.metric} M={self.M} size={len(self)}"
+                + (" read_only" if self.read_only else "") + ">")
 
 
 class BruteForceIndex:

@@ -11,6 +11,8 @@
 * The metric names the binding accepts are exactly those the reference's
   parse_metric accepts (src/dispatch.cpp:15-20), checked against the compiled
   reference.
+* The Python module's surface (properties, membership, repr, simd_isa) passes
+  the reference's own basic-properties test on a host-built index.
 """
 
 from __future__ import annotations
@@ -102,3 +104,27 @@ def test_metric_names_match_the_reference(lib, reference):
         ref_id = reference.parse_metric(name)
         ours = lib.METRICS.get(name, -1)
         assert ours == ref_id, (name, ours, ref_id)
+
+
+def test_module_surface_matches_the_reference(lib, tmp_path):
+    """The reference's test_basic_properties (tests/py/test_index.py:67-76) on
+    a host-built index: sizes, properties, membership, repr and simd_isa."""
+    import workloads
+
+    X = workloads.uniform(3000, 32, seed=3)
+    index = lib.Index(32, "l2", M=16, ef_construction=200)
+    index.add(X, num_threads=2)
+    assert len(index) == 3000
+    assert index.dim == 32
+    assert index.metric == "l2"
+    assert index.M == 16
+    assert index.ef_construction == 200
+    assert not index.read_only
+    assert 0 in index and 2999 in index and 3000 not in index
+    assert "Index dim=32" in repr(index) and "size=3000" in repr(index)
+    assert lib.simd_isa() in {"scalar", "avx2", "avx512", "neon"}
+    assert isinstance(lib.__version__, str)
+    path = tmp_path / "ro.idx"
+    index.save(path)
+    ro = lib.Index.load(path, mmap=True)
+    assert ro.read_only and repr(ro).endswith(" read_only>")
