@@ -130,13 +130,73 @@ def _ptr(a):
 
 
 def _f32(x, dim, what):
-    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
-    single = a.ndim == 1
-    if single:
-        a = a.reshape(1, -1)
-    if a.ndim != 2 or a.shape[1] != dim:
-        raise ValueError(f"{what}: expected shape (n, {dim}) or ({dim},), got {np.shape(x)}")
-    return a, single
+    """(n, dim) C-contiguous float32 copy or view of x, and whether x was one
+    vector. The reference binding's rules (python/bindings.cpp:25-48): any
+    numeric input is cast, anything else is a TypeError, a wrong shape is a
+    ValueError."""
+    try:
+        a = np.asarray(x)
+    except (TypeError, ValueError):
+        a = None
+    if a is None or a.dtype.kind not in "biuf":
+        raise TypeError(f"{what}: expected a numeric array convertible to float32")
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim == 1:
+        if a.shape[0] != dim:
+            raise ValueError(f"{what}: expected {dim} values, got {a.shape[0]}")
+        return a.reshape(1, -1), True
+    if a.ndim != 2:
+        raise ValueError(f"{what}: expected a 1-D or 2-D array")
+    if a.shape[1] != dim:
+        raise ValueError(f"{what}: expected shape (n, {dim}), got ({a.shape[0]}, {a.shape[1]})")
+    return a, False
+
+
+def _labels(ids, n: int, start: int) -> np.ndarray:
+    """uint64 labels of n added vectors: start, start + 1, ... for None, else
+    a 1-D integer array of n non-negative labels (python/bindings.cpp:50-77 of
+    the reference: wrong length ValueError, negative ValueError, non-integer
+    TypeError)."""
+    if ids is None:
+        return np.arange(start, start + n, dtype=np.uint64)
+    try:
+        a = np.asarray(ids)
+    except (TypeError, ValueError):
+        raise TypeError("ids: expected an integer array") from None
+    if a.ndim == 0:
+        a = a.reshape(1)
+    if a.ndim != 1 or a.shape[0] != n:
+        raise ValueError(f"ids: expected shape ({n},)")
+    if a.dtype.kind == "i":
+        if (a < 0).any():
+            raise ValueError("ids: labels must be non-negative")
+    elif a.dtype.kind != "u":
+        raise TypeError("ids: expected an integer array")
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def _search_filter(filter):
+    """The filter argument of Index.search as the reference binding reads it
+    (python/bindings.cpp:79-108): None; a 1-D array, list or tuple that is a
+    bool mask over labels or a list of allowed labels (a BitsetFilter, here
+    its (words, size)); or a callable(label) -> bool (a FunctionFilter)."""
+    if filter is None:
+        return None
+    if isinstance(filter, (np.ndarray, list, tuple)):
+        try:
+            a = np.asarray(filter)
+        except (TypeError, ValueError):
+            a = None
+        if a is None or a.ndim != 1:
+            raise ValueError("filter: expected a 1-D array")
+        if a.dtype.kind == "b":
+            return filter_words(a)
+        if a.dtype.kind in "iu":
+            return filter_words(_labels(a, a.shape[0], 0))
+        raise TypeError("filter: array must be bool (mask over labels) or integer (allowed labels)")
+    if callable(filter):
+        return filter
+    raise TypeError("filter: expected None, a callable(label) -> bool, a bool mask or an id array")
 
 
 LABEL_PREDICATE = C.CFUNCTYPE(C.c_int, C.c_uint64, C.c_void_p)
@@ -514,12 +574,7 @@ class Index:
         num_threads) and returns its statistics as a dict."""
         v, _ = _f32(vectors, self.dim, "vectors")
         n = v.shape[0]
-        if ids is None:
-            labels = np.arange(len(self), len(self) + n, dtype=np.uint64)
-        else:
-            labels = np.ascontiguousarray(np.asarray(ids, dtype=np.uint64).reshape(-1))
-            if labels.size != n:
-                raise ValueError("ids: length must match the number of vectors")
+        labels = _labels(ids, n, len(self))
         if not device:
             _check(_lib.hnsw_index_add_batch(self._h, _ptr(v), _ptr(labels), n, num_threads))
             return None
@@ -561,8 +616,13 @@ class Index:
         return out.reshape(q.shape[0], k)
 
     def search(self, queries, k: int = 10, ef=None, num_threads: int = 0, filter=None):
-        _, single = _f32(queries, self.dim, "queries")
-        rows = self.search_rows(queries, k, ef, filter)
+        """(ids int64, distances float32), (nq, k) or (k,) for one vector; -1 /
+        inf where fewer than k results exist. num_threads is accepted and not
+        used: the GPU runs the whole batch."""
+        if k <= 0:
+            raise ValueError("k must be > 0")
+        q, single = _f32(queries, self.dim, "queries")
+        rows = self.search_rows(q, k, ef, _search_filter(filter))
         ids = rows["label"].astype(np.int64)  # kInvalidLabel -> -1
         d = rows["distance"].copy()
         return (ids[0], d[0]) if single else (ids, d)
