@@ -543,6 +543,10 @@ int hnswgpu_set_ef(hnswgpu_index* h, uint64_t ef) {
 
 namespace hnswamd {
 
+/// Largest batch (bytes of queries and scales in, or of rows and counts out)
+/// that a host-buffer search stages through its context's pinned buffers.
+constexpr size_t kStageBytes = size_t{1} << 20;
+
 /// Host-buffer search used by the C ABI and the C++ facade. Filter words are
 /// label-indexed (a BitsetFilter) or, when by_id, indexed by internal id
 /// (any other Filter, evaluated on the host). Concurrent calls on one handle
@@ -567,15 +571,26 @@ void device_search_host(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t
   auto* d_qs = static_cast<float*>(c.qs.need(nq * h->dim * sizeof(float)));
   auto* d_out = static_cast<hnswgpu_neighbor*>(c.out.need(nq * k * sizeof(hnswgpu_neighbor)));
   auto* d_counts = static_cast<uint32_t*>(c.counts.need(nq * sizeof(uint32_t)));
-  check(hipMemcpyAsync(d_qs, qs, nq * h->dim * sizeof(float), hipMemcpyHostToDevice, st), "hipMemcpy queries");
+  // Small batches go through the context's pinned staging buffers: a copy
+  // from pageable memory costs the runtime a staging pass of its own, which
+  // is most of the copy time of a single query.
+  const size_t q_bytes = nq * h->dim * sizeof(float), s_bytes = nq * sizeof(float);
+  const size_t o_bytes = nq * k * sizeof(hnswgpu_neighbor), c_bytes = nq * sizeof(uint32_t);
+  const bool staged = q_bytes + s_bytes <= kStageBytes && o_bytes + c_bytes <= kStageBytes;
+  uint8_t* in = staged ? static_cast<uint8_t*>(c.stage_in.need(q_bytes + s_bytes)) : nullptr;
+  uint8_t* back = staged ? static_cast<uint8_t*>(c.stage_out.need(o_bytes + c_bytes)) : nullptr;
+  if (staged) std::memcpy(in, qs, q_bytes);
+  check(hipMemcpyAsync(d_qs, staged ? static_cast<const void*>(in) : qs, q_bytes, hipMemcpyHostToDevice, st),
+        "hipMemcpy queries");
   const float* d_scale = nullptr;
   std::vector<float> scales;
   if (h->view.metric == 2) {
     // Cosine: the query scale in double on the host, as the reference computes it.
-    scales.resize(nq);
-    dev::host_query_scales(qs, nq, h->dim, scales.data());
-    auto* f = static_cast<float*>(c.qscale.need(nq * sizeof(float)));
-    check(hipMemcpyAsync(f, scales.data(), nq * sizeof(float), hipMemcpyHostToDevice, st), "hipMemcpy scales");
+    if (!staged) scales.resize(nq);
+    float* sc = staged ? reinterpret_cast<float*>(in + q_bytes) : scales.data();
+    dev::host_query_scales(qs, nq, h->dim, sc);
+    auto* f = static_cast<float*>(c.qscale.need(s_bytes));
+    check(hipMemcpyAsync(f, sc, s_bytes, hipMemcpyHostToDevice, st), "hipMemcpy scales");
     d_scale = f;
   }
   const uint64_t* d_fw = nullptr;
@@ -596,13 +611,18 @@ void device_search_host(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t
     allowed = bits > 0 ? static_cast<double>(ones) / static_cast<double>(bits) : 0.0;
   }
   search_device(*h, c, d_qs, d_scale, nq, k, ef, d_fw, filter_size, by_id, d_out, d_counts, st, allowed);
-  check(hipMemcpyAsync(out, d_out, nq * k * sizeof(hnswgpu_neighbor), hipMemcpyDeviceToHost, st),
+  check(hipMemcpyAsync(staged ? static_cast<void*>(back) : out, d_out, o_bytes, hipMemcpyDeviceToHost, st),
         "hipMemcpy results");
   if (counts != nullptr) {
-    check(hipMemcpyAsync(counts, d_counts, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "hipMemcpy counts");
+    check(hipMemcpyAsync(staged ? static_cast<void*>(back + o_bytes) : counts, d_counts, c_bytes,
+                         hipMemcpyDeviceToHost, st), "hipMemcpy counts");
   }
   dev::ctx_end(c, st);
   check(hipStreamSynchronize(st), "search_batch");
+  if (staged) {
+    std::memcpy(out, back, o_bytes);
+    if (counts != nullptr) std::memcpy(counts, back + o_bytes, c_bytes);
+  }
 }
 
 /// Device-buffer search (asynchronous on `stream`).

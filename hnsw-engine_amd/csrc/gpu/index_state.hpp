@@ -117,6 +117,7 @@ struct CallCtx {
   Scratch sc;
   DevBuf qs, out, counts, filter, qscale, task_dev;
   PinnedBuf task_host;
+  PinnedBuf stage_in, stage_out;  // host-buffer searches of small batches: queries + scales, rows + counts
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   hipStream_t last_stream = nullptr;
