@@ -568,6 +568,15 @@ void device_search_host(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t
   dev::CallCtx& c = *lease.c;
   hipStream_t st = c.stream;
   dev::ctx_begin(c, st);
+  // A call that ends in an exception waits for the copies it queued before
+  // the context (and its pinned staging buffers) goes to the next caller.
+  struct DrainOnError {
+    hipStream_t st;
+    bool armed = true;
+    ~DrainOnError() {
+      if (armed) (void)hipStreamSynchronize(st);
+    }
+  } drain{st};
   auto* d_qs = static_cast<float*>(c.qs.need(nq * h->dim * sizeof(float)));
   auto* d_out = static_cast<hnswgpu_neighbor*>(c.out.need(nq * k * sizeof(hnswgpu_neighbor)));
   auto* d_counts = static_cast<uint32_t*>(c.counts.need(nq * sizeof(uint32_t)));
@@ -619,6 +628,7 @@ void device_search_host(hnswgpu_index* h, const float* qs, uint64_t nq, uint64_t
   }
   dev::ctx_end(c, st);
   check(hipStreamSynchronize(st), "search_batch");
+  drain.armed = false;
   if (staged) {
     std::memcpy(out, back, o_bytes);
     if (counts != nullptr) std::memcpy(counts, back + o_bytes, c_bytes);

@@ -15,6 +15,8 @@
   before stay valid, keep their options, and search the new graph exactly.
 * Device-buffer cosine searches prepare queries with the integer emulation of
   the reference's double arithmetic and still return the golden rows.
+* A host-buffer search that fails after queueing its copies leaves its call
+  context usable: larger searches afterwards return the oracle's rows.
 """
 
 from __future__ import annotations
@@ -243,3 +245,27 @@ def test_device_buffer_cosine_equals_golden_rows(golden, lib):
     want = golden["cos_d100/avx512/plain/k10_ef64"]
     for i, (rows, n) in enumerate(res):
         assert rows_equal(rows, want[i]), i
+
+
+def test_failed_call_then_larger_calls_stay_exact(lib, oracle, tmp_path):
+    """Host-buffer searches stage small batches through pinned buffers of the
+    call's context. A call that fails after queueing its copies (ef beyond 32
+    bits) must leave that context usable: the following calls, whose batches
+    grow the staging buffers, return the oracle's rows."""
+    import workloads
+
+    X = workloads.mixture_128(8000, seed=71)
+    Q = workloads.mixture_128(1200, seed=1071)
+    ix = lib.Index(128, "l2")
+    ix.add(X, num_threads=0)
+    path = tmp_path / "m8k.idx"
+    ix.save(path)
+    orc = oracle.load(path)
+    dev = ix.device()
+    for nq in (1, 300, 1200):
+        print("case nq", nq, flush=True)
+        with pytest.raises(ValueError, match="out of range"):
+            dev.search_rows(Q[:nq], 10, 1 << 33)
+        got, counts = dev.search_rows(Q[:nq], 10, 64)
+        assert rows_equal(got, orc.search(Q[:nq], 10, 64, order="avx512")), nq
+        assert (counts == 10).all()
